@@ -1,0 +1,149 @@
+"""Benchmark of the coupled LBM step on one MI355X.
+
+Workload: 256^3 single scene (liquid 40%, perturbed interface, falling sphere
+r=24), float32 populations, FP64 arithmetic, one coupled step = sim.step;
+absorbDeadCells; sphere step.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 50] [--size 256]
+
+MLUPS counts all nx*ny*nz cells, walls included.  The roofline object is for
+the dominant kernel (stream + macro + next collide), timed with HIP events on
+the library's stream inside the timed region; its algorithmic traffic is 297 B
+per cell (27+7 populations read and written, phi(t) and phi(t+1) read, rho and
+u written, tag read).  The CPU baseline is the oracle (a single-threaded C
+restatement of the reference) on the same scene for a few steps."""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "claude-vibe-code-raytracing_amd")
+sys.path.insert(0, PKG)
+
+import lbm  # noqa: E402
+import scenegen  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_LU = {"stream_collide": 297, "phase_stream": 33, "stream_macro": 297}
+METRIC = "coupled MLUPS + %HBM peak at 256^3; served scenes MLUPS, p99 frame latency"
+
+
+def build_scene(n, seed):
+    params, yw, sph = scenegen.config1(n, n, n, seed)
+    sim = lbm.Sim(n, n, n)
+    sim.set_params(**params)
+    sim.init_waterline_map(yw)
+    b = lbm.make_sphere(n, n, n, sph["cx"], sph["cy"], sph["cz"], sph["r"], sph["density"])
+    b.vx, b.vy, b.vz = sph["vx"], sph["vy"], sph["vz"]
+    return sim, b, params, yw, sph
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(n, seed, steps):
+    """The oracle (single thread) on the same scene: MLUPS over `steps` steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+    params, yw, sph = scenegen.config1(n, n, n, seed)
+    s = O.OSim(n, n, n)
+    s.set_params(**params)
+    s.init_waterline_map(yw)
+    b = O.sphere(n, n, n, sph["cx"], sph["cy"], sph["cz"], sph["r"], sph["density"])
+    b.vx, b.vy, b.vz = sph["vx"], sph["vy"], sph["vz"]
+    s.coupled(b, 1)  # first step includes the initial dead-cell sweep
+    t0 = time.perf_counter()
+    s.coupled(b, steps)
+    dt = time.perf_counter() - t0
+    return {"value": round(n ** 3 * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": 1,
+            "kind": "port",
+            "sample": f"{n}^3 config-1 scene, {steps} coupled steps after 1 untimed, oracle "
+                      f"(C restatement of sim.js/solid.js, FP64 arithmetic) on 1 thread of "
+                      f"{cpu_model()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs the coupled step on a single MI355X "
+              "(multi-GPU sharding is a design note only).", file=sys.stderr)
+        return 2
+    n = args.size
+    sim, b, params, yw, sph = build_scene(n, args.seed)
+    print(f"[bench] scene {n}^3 built; warmup {args.warmup}", file=sys.stderr, flush=True)
+    sim.coupled(b, args.warmup)
+    sim.synchronize()
+    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    sim.set_timing(True)
+    t0 = time.perf_counter()
+    sim.coupled(b, args.steps)
+    sim.synchronize()
+    dt = time.perf_counter() - t0
+    kt = {k: sim.kernel_time(k) for k in ("stream_collide", "phase_stream", "stream_macro",
+                                          "body", "band", "collide_all")}
+    sim.set_timing(False)
+    cells = n ** 3
+    mlups = cells * args.steps / dt / 1e6
+    ms_sc, cnt_sc = kt["stream_collide"]
+    avg_s = ms_sc / max(cnt_sc, 1) / 1e3
+    achieved = BYTES_PER_LU["stream_collide"] * cells / avg_s / 1e9 if cnt_sc else None
+    # sanity: the state stays finite
+    phi = sim.get("phi")
+    finite = bool((phi == phi).all())
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"config 1: {n}^3 single scene, coupled step (D3Q27 hydro + D3Q7 "
+                               f"phase + sphere), float32 populations", "nx": n, "ny": n, "nz": n,
+                   "storage": "f32", "arithmetic": "f64", "seed": args.seed,
+                   "sphere_r": sph["r"], "params": params},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": None, "kernel": "k_stream_collide",
+                     "bytes_per_lu": BYTES_PER_LU["stream_collide"],
+                     "avg_ms": round(avg_s * 1e3, 4)},
+        "whole_step_hbm_frac": round(297 * mlups * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
+        "kernel_ms": {k: {"total": round(v[0], 3), "launches": v[1]} for k, v in kt.items()},
+        "finite": finite,
+        "sphere": {"cx": b.cx, "cy": b.cy, "cz": b.cz, "vx": b.vx, "vy": b.vy, "vz": b.vz},
+    }
+    sim.close()
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(n, args.seed, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

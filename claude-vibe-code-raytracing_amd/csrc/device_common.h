@@ -1,0 +1,59 @@
+// Device-side types shared by every kernel of liblbm.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lattice.h"
+
+namespace lbm {
+
+// Grid geometry.  Populations are structure-of-arrays: population k of cell i is
+// at k*S + i, with the plane stride S = n rounded up to 64 cells (256 bytes), so
+// every plane starts on an HBM burst boundary.
+struct Geom {
+  int nx, ny, nz, nxny;
+  int64_t n, S;
+};
+
+// Per-step constants, computed on the host from lbm_params with the same
+// double-precision expressions the reference evaluates at the top of each pass.
+struct Derived {
+  // collideHydro / streamHydroAndMacro (src/sim.js:410-416, 490-492)
+  double invTau, beta, kappa, dRho, A, rhoG, gravity;
+  // collidePhase (src/sim.js:327-332)
+  double invTauPhi, sharpC, sharpFac;
+};
+
+// Device control block: list counters, error flags, the device copy of the
+// sphere and the band boxes of the fused step.
+struct Ctrl {
+  int n_fresh, n_dead, n_solid, n_links;
+  int err;         // nonzero: a device-side check failed (see kErr*)
+  int n_partials;  // blocks of the fused stream kernel that touched the body
+  int box_band[6];   // x0,x1,y0,y1,z0,z1 (inclusive) of the band handled by the band kernels
+  int box_fused[6];  // band box that the next fused launch must skip
+  lbm_sphere sphere; // device sphere for the fused coupled step
+  double F_links[6]; // link momentum of the last fused step (Fx,Fy,Fz,Tx,Ty,Tz), diagnostics
+};
+
+constexpr int kErrListOverflow = 1;
+constexpr int kErrBandEscape = 2;
+constexpr int kErrFreshCapacity = 4;
+
+__device__ __forceinline__ int64_t cell_index(const Geom &g, int x, int y, int z) {
+  return ((int64_t)z * g.ny + y) * g.nx + x;
+}
+
+__device__ __forceinline__ bool interior(const Geom &g, int x, int y, int z) {
+  return x >= 1 && x < g.nx - 1 && y >= 1 && y < g.ny - 1 && z >= 1 && z < g.nz - 1;
+}
+
+// Pull offset of direction k: the source of cell i is i + off = i - e_k
+// (src/sim.js:71-78).
+template <int K>
+__device__ __forceinline__ int64_t pull_off27(const Geom &g) {
+  return -((int64_t)EZ27[K] * g.nxny + (int64_t)EY27[K] * g.nx + EX27[K]);
+}
+
+}  // namespace lbm

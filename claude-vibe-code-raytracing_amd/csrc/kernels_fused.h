@@ -1,0 +1,645 @@
+// Kernels of the fused coupled step (lbm_step).
+//
+// The reference's step t runs retag(t), fresh(t), gradients, collide(t),
+// stream(t) and the macroscopic update, then the body absorbs dead cells and
+// moves.  The fused form reorders nothing; it regroups the passes around the
+// only global dependency, the sphere position, which needs the momentum sum of
+// stream(t) before retag(t+1) can run:
+//
+//   k_phase_stream(t)     phi(t+1) = clamp(sum of pulled h) for every cell
+//   k_stream_collide(t)   stream(t) + macro(t) for every fluid cell, then
+//                         collide(t+1) for every fluid cell outside the band;
+//                         band cells keep their post-stream populations
+//   k_body(t)             momentum sums in a fixed order, dead-cell impulses,
+//                         sphere step, next band boxes
+//   k_retag(t+1, band)    retag inside the band box only
+//   k_fresh(t+1)          fresh cells, in the reference's order
+//   k_collide_region(t+1, band)  gradients + collide for band cells
+//
+// The band box holds every cell whose tag can change at t+1, every fresh cell
+// and its face neighbours (the only cells whose collide(t+1) reads values the
+// fresh pass rewrites), and every cell changed at t.  The sphere moves at most
+// kBandDelta cells per axis per step once it is inside its clamp box
+// (src/solid.js:159-164, 177-179, 199-205), so a box of radius r + kBandDelta + 2
+// around the current centre covers all of them.  Outside the band, tags cannot
+// change and no fresh cell is in reach, so collide(t+1) there reads exactly what
+// the reference's collide(t+1) reads.
+#pragma once
+
+#include "device_common.h"
+
+namespace lbm {
+
+// Largest per-axis displacement of the sphere centre in one step while it is
+// inside its clamp box: |v| <= 0.08 after the speed cap, times 0.98 damping.
+constexpr double kBandDelta = 0.09;
+constexpr int kBandMargin = 2;
+
+// Axis-aligned band box around a sphere, clamped to the grid.
+__host__ __device__ inline void band_box(const lbm_sphere &b, int nx, int ny, int nz, int box[6]) {
+  const double c[3] = {b.cx, b.cy, b.cz};
+  const int n[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; a++) {
+    const double lo = c[a] - b.r - 0.5 - kBandDelta;
+    const double hi = c[a] + b.r - 0.5 + kBandDelta;
+    int l, h;
+    if (!(lo == lo) || !(hi == hi) || lo < -1e9 || hi > 1e9) {
+      l = 0;
+      h = n[a] - 1;
+    } else {
+      l = (int)floor(lo) - kBandMargin;
+      h = (int)ceil(hi) + kBandMargin;
+    }
+    box[2 * a] = l < 0 ? 0 : l;
+    box[2 * a + 1] = h > n[a] - 1 ? n[a] - 1 : h;
+  }
+}
+
+__device__ __forceinline__ bool in_box(const int box[6], int x, int y, int z) {
+  return x >= box[0] && x <= box[1] && y >= box[2] && y <= box[3] && z >= box[4] && z <= box[5];
+}
+
+// phi(t+1) for every cell: fluid cells sum their pulled h (src/sim.js:376-397,
+// 549-552); other cells keep phi (the reference never updates them).
+__global__ __launch_bounds__(256) void k_phase_stream(Geom g, const float *__restrict__ h2,
+                                                      const uint8_t *__restrict__ tag,
+                                                      const float *__restrict__ phi,
+                                                      float *__restrict__ phi_next) {
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * 4 + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= g.nx || y >= g.ny) return;
+  const int64_t i = cell_index(g, x, y, z);
+  if (tag[i] != TAG_FLUID || !interior(g, x, y, z)) {
+    phi_next[i] = phi[i];
+    return;
+  }
+  // All source tags first, then every load with its address already chosen, so
+  // the seven loads are in flight together.
+  uint32_t nonfluid = 0;
+#pragma unroll
+  for (int k = 1; k < Q7; k++) {
+    const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
+    nonfluid |= (uint32_t)(tag[s] != TAG_FLUID) << k;
+  }
+  float hv[Q7];
+#pragma unroll
+  for (int k = 0; k < Q7; k++) {
+    const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
+    const int64_t a = ((nonfluid >> k) & 1u) ? opp7(k) * g.S + i : k * g.S + s;
+    hv[k] = h2[a];
+  }
+  double phiSum = 0;
+#pragma unroll
+  for (int k = 0; k < Q7; k++) phiSum += (double)hv[k];
+  const double p = phiSum < 0 ? 0 : phiSum > 1 ? 1 : phiSum;
+  phi_next[i] = (float)p;
+}
+
+// Gradient and Laplacian of a phi array at interior cell i (src/sim.js:186-195),
+// rounded to float as the reference's gx/gy/gz/lap arrays store them.
+__device__ __forceinline__ void phi_grad(const Geom &g, const float *__restrict__ phi, int64_t i,
+                                         double c, float &gxo, float &gyo, float &gzo,
+                                         float &lapo) {
+  const double px = phi[i + 1], mx = phi[i - 1];
+  const double py = phi[i + g.nx], my = phi[i - g.nx];
+  const double pz = phi[i + g.nxny], mz = phi[i - g.nxny];
+  gxo = (float)(0.5 * (px - mx));
+  gyo = (float)(0.5 * (py - my));
+  gzo = (float)(0.5 * (pz - mz));
+  lapo = (float)(px + mx + py + my + pz + mz - 6 * c);
+}
+
+// e_k . v for a lattice direction with components in {-1, 0, 1}.  The reference
+// evaluates ekx*vx + eky*vy + ekz*vz; a zero component adds a signed zero, which
+// leaves every nonzero sum unchanged, so only the nonzero terms are summed, in
+// the same x, y, z order.
+template <int EXK, int EYK, int EZK>
+__device__ __forceinline__ double edot(double vx, double vy, double vz) {
+  double s = 0;
+  bool first = true;
+  if (EXK != 0) {
+    s = EXK > 0 ? vx : -vx;
+    first = false;
+  }
+  if (EYK != 0) {
+    const double t = EYK > 0 ? vy : -vy;
+    s = first ? t : (EYK > 0 ? s + vy : s - vy);
+    first = false;
+  }
+  if (EZK != 0) {
+    const double t = EZK > 0 ? vz : -vz;
+    s = first ? t : (EZK > 0 ? s + vz : s - vz);
+    first = false;
+  }
+  return first ? 0.0 : s;
+}
+
+// collidePhase for one cell (src/sim.js:341-360).
+__device__ __forceinline__ void collide_phase_cell(const Derived &d, double p, double uxi,
+                                                   double uyi, double uzi, double ggx, double ggy,
+                                                   double ggz, const float hin[Q7],
+                                                   float hout[Q7]) {
+  const double gmag = sqrt(ggx * ggx + ggy * ggy + ggz * ggz) + 1e-12;
+  const double sharpen = d.sharpC * p * (1 - p);
+  const double Sx = sharpen * d.sharpFac * ggx / gmag;
+  const double Sy = sharpen * d.sharpFac * ggy / gmag;
+  const double Sz = sharpen * d.sharpFac * ggz / gmag;
+  const double wp0 = (double)w7(0) * p;
+  const double wp1 = (double)w7(1) * p;
+  const double it = d.invTauPhi;
+  // k = 0: eu = 0, source 0.
+  {
+    const double heq = wp0 * (1 + INV_CS2_7 * 0.0);
+    const double hk = hin[0];
+    hout[0] = (float)(hk - it * (hk - heq) + (double)w7(0) * 0.0);
+  }
+  const double u[3] = {uxi, uyi, uzi};
+  const double S[3] = {Sx, Sy, Sz};
+#pragma unroll
+  for (int k = 1; k < Q7; k++) {
+    const int a = (k - 1) >> 1;           // axis of direction k
+    const bool pos = (k & 1) != 0;        // +1 for odd k
+    const double eu = pos ? u[a] : -u[a];
+    const double heq = wp1 * (1 + INV_CS2_7 * eu);
+    const double Fk = (double)w7(k) * (pos ? S[a] : -S[a]);
+    const double hk = hin[k];
+    hout[k] = (float)(hk - it * (hk - heq) + Fk);
+  }
+}
+
+// collideHydro for one cell (src/sim.js:430-459).
+template <int K>
+__device__ __forceinline__ float collide_hydro_dir(const Derived &d, double sumF, double rhoPhase,
+                                                   double negU2half, double uF, double uxe,
+                                                   double uye, double uze, double fx, double fy,
+                                                   double fz, float fin) {
+  constexpr int ex = EX27[K], ey = EY27[K], ez = EZ27[K];
+  const double wk = w27(K);
+  const double eu = edot<ex, ey, ez>(uxe, uye, uze);
+  const double Qi = negU2half + eu * (INV_CS2_27 + INV_2CS4_27 * eu);
+  const double feq = wk * (sumF + rhoPhase * Qi);
+  const double ekF = edot<ex, ey, ez>(fx, fy, fz);
+  const double Sk = d.A * wk * (ekF * (1 + INV_CS2_27 * eu) - uF);
+  const double fk = fin;
+  return (float)(fk - d.invTau * (fk - feq) + Sk);
+}
+
+template <int K>
+struct HydroUnroll {
+  __device__ __forceinline__ static void run(const Derived &d, double sumF, double rhoPhase,
+                                             double negU2half, double uF, double uxe, double uye,
+                                             double uze, double fx, double fy, double fz,
+                                             const float *fin, float *fout) {
+    HydroUnroll<K - 1>::run(d, sumF, rhoPhase, negU2half, uF, uxe, uye, uze, fx, fy, fz, fin, fout);
+    fout[K - 1] = collide_hydro_dir<K - 1>(d, sumF, rhoPhase, negU2half, uF, uxe, uye, uze, fx,
+                                           fy, fz, fin[K - 1]);
+  }
+};
+template <>
+struct HydroUnroll<0> {
+  __device__ __forceinline__ static void run(const Derived &, double, double, double, double,
+                                             double, double, double, double, double, double,
+                                             const float *, float *) {}
+};
+
+__device__ __forceinline__ void collide_hydro_cell(const Derived &d, double sumF, double p,
+                                                   double lap, double ggx, double ggy, double ggz,
+                                                   double bx, double by, double bz, double uxe,
+                                                   double uye, double uze, const float fin[Q27],
+                                                   float fout[Q27]) {
+  const double rhoPhase = d.rhoG + p * d.dRho;
+  const double mu = 4 * d.beta * p * (p - 1) * (2 * p - 1) - d.kappa * lap;
+  const double fx = bx + mu * ggx;
+  const double fy = by + mu * ggy + rhoPhase * d.gravity;
+  const double fz = bz + mu * ggz;
+  const double u2 = uxe * uxe + uye * uye + uze * uze;
+  const double negU2half = -1.5 * u2;
+  const double uF = uxe * fx + uye * fy + uze * fz;
+  HydroUnroll<Q27>::run(d, sumF, rhoPhase, negU2half, uF, uxe, uye, uze, fx, fy, fz, fin, fout);
+}
+
+// Pull stream of the 27 populations with halfway bounce-back
+// (src/sim.js:509-543).  All source tags are read first and every load address
+// is chosen without branching (own opposite population at a non-fluid source),
+// so the 27 loads are in flight together; the moving-wall term and the momentum
+// exchange run only for solid links.  fk stays the unrounded double the
+// reference sums and exchanges; fp receives it rounded to float.
+__device__ __forceinline__ void pull27(const Geom &g, int64_t i, const float *__restrict__ f2,
+                                       const uint8_t *__restrict__ tag,
+                                       const float *__restrict__ usx,
+                                       const float *__restrict__ usy,
+                                       const float *__restrict__ usz, double rhoPhaseOld,
+                                       bool exchange, int x, int y, int z, double cx, double cy,
+                                       double cz, double mom[6], bool &linked, float fp[Q27],
+                                       double &pSumF, double &mx, double &my, double &mz,
+                                       uint32_t &nonfluid) {
+  nonfluid = 0;
+  uint32_t solid = 0;
+#pragma unroll
+  for (int k = 1; k < Q27; k++) {
+    const int64_t s = i - ((int64_t)EZ27[k] * g.nxny + (int64_t)EY27[k] * g.nx + EX27[k]);
+    const uint8_t st = tag[s];
+    nonfluid |= (uint32_t)(st != TAG_FLUID) << k;
+    solid |= (uint32_t)(st == TAG_SOLID) << k;
+  }
+  float raw[Q27];
+#pragma unroll
+  for (int k = 0; k < Q27; k++) {
+    const int64_t s = i - ((int64_t)EZ27[k] * g.nxny + (int64_t)EY27[k] * g.nx + EX27[k]);
+    const int64_t a = ((nonfluid >> k) & 1u) ? opp27(k) * g.S + i : k * g.S + s;
+    raw[k] = f2[a];
+  }
+  pSumF = 0;
+  mx = my = mz = 0;
+#pragma unroll
+  for (int k = 0; k < Q27; k++) {
+    double fk = (double)raw[k];
+    if ((solid >> k) & 1u) {
+      const int ex = EX27[k], ey = EY27[k], ez = EZ27[k];
+      const int64_t s = i - ((int64_t)ez * g.nxny + (int64_t)ey * g.nx + ex);
+      const double fopp = fk;
+      const double eu = (double)ex * (double)usx[s] + (double)ey * (double)usy[s] +
+                        (double)ez * (double)usz[s];
+      const double corr = 2 * (double)w27(k) * rhoPhaseOld * eu * INV_CS2_27;
+      fk = fopp + corr;
+      if (exchange) {
+        // applyImpulseAtCell(x - ex, y - ey, z - ez, -m ex, -m ey, -m ez)
+        const double m = fopp + fk;
+        const double px = -m * ex, py = -m * ey, pz = -m * ez;
+        const double rx = (x - ex) + 0.5 - cx;
+        const double ry = (y - ey) + 0.5 - cy;
+        const double rz = (z - ez) + 0.5 - cz;
+        mom[0] += px;
+        mom[1] += py;
+        mom[2] += pz;
+        mom[3] += ry * pz - rz * py;
+        mom[4] += rz * px - rx * pz;
+        mom[5] += rx * py - ry * px;
+        linked = true;
+      }
+    }
+    fp[k] = (float)fk;
+    pSumF += fk;
+    // mx += ekx * fk: a zero component adds a signed zero to a sum that starts at
+    // +0, which changes nothing, so only nonzero components are accumulated.
+    if (EX27[k] > 0) mx += fk;
+    if (EX27[k] < 0) mx += -fk;
+    if (EY27[k] > 0) my += fk;
+    if (EY27[k] < 0) my += -fk;
+    if (EZ27[k] > 0) mz += fk;
+    if (EZ27[k] < 0) mz += -fk;
+  }
+}
+
+// Deterministic block sum of 6 doubles (fixed butterfly within each wave, waves
+// in order), written to partials[block] and the block id appended to the list.
+__device__ __forceinline__ void block_momentum(double mom[6], bool linked, double *partials,
+                                               int32_t *plist, int *pcount, int64_t block) {
+  __shared__ double red[4][6];
+  const int lane = threadIdx.x;  // 0..63
+  const int wave = threadIdx.y;  // 0..3
+  if (!__syncthreads_or(linked ? 1 : 0)) return;
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    double v = mom[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    mom[c] = v;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < 6; c++) red[wave][c] = mom[c];
+  }
+  __syncthreads();
+  if (lane == 0 && wave == 0) {
+#pragma unroll
+    for (int c = 0; c < 6; c++)
+      partials[block * 6 + c] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
+    const int slot = atomicAdd(pcount, 1);
+    plist[slot] = (int32_t)block;
+  }
+}
+
+// stream(t) + macro(t) for every fluid cell; with COLLIDE, also collide(t+1)
+// for fluid cells outside the band.  Without COLLIDE (the last step of a call)
+// the post-stream populations are stored for every cell, which leaves the state
+// exactly as the reference's step does, and phi(t+1) is written here.
+//
+// f2/h2: post-collision populations of step t.  fo/ho: output planes.
+// phi: phi(t) after fresh(t).  phi_next: phi(t+1) (read with COLLIDE, written
+// without).
+template <bool COLLIDE>
+__global__ __launch_bounds__(256) void k_stream_collide(
+    Geom g, Derived d, const float *__restrict__ f2, const float *__restrict__ h2,
+    float *__restrict__ fo, float *__restrict__ ho, const float *__restrict__ phi,
+    float *__restrict__ phi_next, float *__restrict__ rho, float *__restrict__ ux,
+    float *__restrict__ uy, float *__restrict__ uz, const uint8_t *__restrict__ tag,
+    const float *__restrict__ usx, const float *__restrict__ usy, const float *__restrict__ usz,
+    const float *__restrict__ bfx, const float *__restrict__ bfy, const float *__restrict__ bfz,
+    int use_bf, const Ctrl *__restrict__ ctrl, int has_solid, double *__restrict__ partials,
+    int32_t *__restrict__ plist, int *__restrict__ pcount) {
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * 4 + threadIdx.y;
+  const int z = blockIdx.z;
+  double mom[6] = {0, 0, 0, 0, 0, 0};
+  bool linked = false;
+  const bool exchange = has_solid != 0;
+  double cx = 0, cy = 0, cz = 0;
+  if (exchange) {
+    cx = ctrl->sphere.cx;
+    cy = ctrl->sphere.cy;
+    cz = ctrl->sphere.cz;
+  }
+  if (x < g.nx && y < g.ny) {
+    const int64_t i = cell_index(g, x, y, z);
+    const bool fluid = tag[i] == TAG_FLUID && interior(g, x, y, z);
+    if (!fluid) {
+      if (!COLLIDE) phi_next[i] = phi[i];
+    } else {
+      const double phiOld = phi[i];
+      const double rhoPhaseOld = d.rhoG + phiOld * d.dRho;
+      float gxo, gyo, gzo, lapo;
+      phi_grad(g, phi, i, phiOld, gxo, gyo, gzo, lapo);
+      float fp[Q27];
+      double pSumF, mx, my, mz;
+      uint32_t nonfluid;
+      pull27(g, i, f2, tag, usx, usy, usz, rhoPhaseOld, exchange, x, y, z, cx, cy, cz, mom, linked,
+             fp, pSumF, mx, my, mz, nonfluid);
+      const double rhoN = pSumF > 0.01 ? pSumF : 0.01;
+      // D3Q7 directions 1..6 are D3Q27 directions 1..6, so the same source mask
+      // selects their bounce-back.
+      float hp[Q7];
+#pragma unroll
+      for (int k = 0; k < Q7; k++) {
+        const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
+        const int64_t a = ((nonfluid >> k) & 1u) ? opp7(k) * g.S + i : k * g.S + s;
+        hp[k] = h2[a];
+      }
+      double phiSum = 0;
+#pragma unroll
+      for (int k = 0; k < Q7; k++) phiSum += (double)hp[k];
+      const double p = phiSum < 0 ? 0 : phiSum > 1 ? 1 : phiSum;
+      const double rhoPhaseNew = d.rhoG + p * d.dRho;
+      const double mu = 4 * d.beta * p * (p - 1) * (2 * p - 1) - d.kappa * (double)lapo;
+      const double bx = use_bf ? (double)bfx[i] : 0.0;
+      const double by = use_bf ? (double)bfy[i] : 0.0;
+      const double bz = use_bf ? (double)bfz[i] : 0.0;
+      const double fx = bx + mu * (double)gxo;
+      const double fy = by + mu * (double)gyo + rhoPhaseNew * d.gravity;
+      const double fz = bz + mu * (double)gzo;
+      const double invRho = 1 / rhoPhaseNew;
+      double uxn = (mx + 0.5 * fx) * invRho;
+      double uyn = (my + 0.5 * fy) * invRho;
+      double uzn = (mz + 0.5 * fz) * invRho;
+      const double u2 = uxn * uxn + uyn * uyn + uzn * uzn;
+      const double UMAX = 0.2, UMAX2 = UMAX * UMAX;
+      if (u2 > UMAX2) {
+        const double sc = UMAX / sqrt(u2);
+        uxn *= sc;
+        uyn *= sc;
+        uzn *= sc;
+      }
+      const float rhof = (float)rhoN, phif = (float)p;
+      const float uxf = (float)uxn, uyf = (float)uyn, uzf = (float)uzn;
+      rho[i] = rhof;
+      ux[i] = uxf;
+      uy[i] = uyf;
+      uz[i] = uzf;
+      bool store_stream = true;
+      if (COLLIDE) {
+        int box[6];
+#pragma unroll
+        for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
+        if (!in_box(box, x, y, z)) {
+          store_stream = false;
+          // collide(t+1) with the new macroscopic values (rounded to float as the
+          // reference stores them) and the gradients of phi(t+1).
+          float gx1, gy1, gz1, lap1;
+          phi_grad(g, phi_next, i, (double)phif, gx1, gy1, gz1, lap1);
+          float hc[Q7];
+          collide_phase_cell(d, (double)phif, (double)uxf, (double)uyf, (double)uzf, (double)gx1,
+                             (double)gy1, (double)gz1, hp, hc);
+#pragma unroll
+          for (int k = 0; k < Q7; k++) ho[k * g.S + i] = hc[k];
+          float fc[Q27];
+          collide_hydro_cell(d, (double)rhof, (double)phif, (double)lap1, (double)gx1, (double)gy1,
+                             (double)gz1, 0.0, 0.0, 0.0, (double)uxf, (double)uyf, (double)uzf, fp,
+                             fc);
+#pragma unroll
+          for (int k = 0; k < Q27; k++) fo[k * g.S + i] = fc[k];
+        }
+      } else {
+        phi_next[i] = phif;
+      }
+      if (store_stream) {
+#pragma unroll
+        for (int k = 0; k < Q27; k++) fo[k * g.S + i] = fp[k];
+#pragma unroll
+        for (int k = 0; k < Q7; k++) ho[k * g.S + i] = hp[k];
+      }
+    }
+  }
+  if (exchange) {
+    const int64_t block = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    block_momentum(mom, linked, partials, plist, pcount, block);
+  }
+}
+
+// Gradients + collide for the fluid cells of a box, reading post-stream
+// populations and writing post-collision ones (in place when fi == fo).  Used for
+// the whole grid at the first step of a call and for the band afterwards.
+__global__ __launch_bounds__(256) void k_collide_region(
+    Geom g, Derived d, const float *fi, float *fo, const float *hi, float *ho,
+    const float *__restrict__ phi, const float *__restrict__ rho, const float *__restrict__ ux,
+    const float *__restrict__ uy, const float *__restrict__ uz, const uint8_t *__restrict__ tag,
+    const float *__restrict__ bfx, const float *__restrict__ bfy, const float *__restrict__ bfz,
+    int use_bf, const Ctrl *__restrict__ ctrl, int use_ctrl_box) {
+  int box[6] = {0, g.nx - 1, 0, g.ny - 1, 0, g.nz - 1};
+  if (use_ctrl_box) {
+#pragma unroll
+    for (int j = 0; j < 6; j++) box[j] = ctrl->box_band[j];
+  }
+  const int x = box[0] + blockIdx.x * 64 + threadIdx.x;
+  const int y = box[2] + blockIdx.y * 4 + threadIdx.y;
+  const int z = box[4] + blockIdx.z;
+  if (x > box[1] || y > box[3] || z > box[5]) return;
+  if (!interior(g, x, y, z)) return;
+  const int64_t i = cell_index(g, x, y, z);
+  if (tag[i] != TAG_FLUID) return;
+  const double p = phi[i];
+  float gx1, gy1, gz1, lap1;
+  phi_grad(g, phi, i, p, gx1, gy1, gz1, lap1);
+  const double uxe = ux[i], uye = uy[i], uze = uz[i];
+  float hin[Q7], hc[Q7];
+#pragma unroll
+  for (int k = 0; k < Q7; k++) hin[k] = hi[k * g.S + i];
+  collide_phase_cell(d, p, uxe, uye, uze, (double)gx1, (double)gy1, (double)gz1, hin, hc);
+#pragma unroll
+  for (int k = 0; k < Q7; k++) ho[k * g.S + i] = hc[k];
+  float fin[Q27], fc[Q27];
+#pragma unroll
+  for (int k = 0; k < Q27; k++) fin[k] = fi[k * g.S + i];
+  const double bx = use_bf ? (double)bfx[i] : 0.0;
+  const double by = use_bf ? (double)bfy[i] : 0.0;
+  const double bz = use_bf ? (double)bfz[i] : 0.0;
+  collide_hydro_cell(d, (double)rho[i], p, (double)lap1, (double)gx1, (double)gy1, (double)gz1, bx,
+                     by, bz, uxe, uye, uze, fin, fc);
+#pragma unroll
+  for (int k = 0; k < Q27; k++) fo[k * g.S + i] = fc[k];
+}
+
+// Deterministic sum of one double per thread over a 1024-thread block: pairs are
+// combined in a fixed tree, so the result does not depend on scheduling.
+__device__ __forceinline__ double block_tree_sum(double v, double *red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (tid < w) red[tid] = red[tid] + red[tid + w];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// Body update after stream(t), one 1024-thread workgroup:
+//   F/T += link momentum, summed over the blocks that touched the body in
+//          ascending block order (contiguous chunks per thread, fixed tree);
+//   F/T += dead-cell impulses rho_phase(phi) u at each cell that turned solid at
+//          this step's retag (src/solid.js:87-101), same chunked fixed order;
+//   RigidSphere.step(1, g) (src/solid.js:104-210).
+// Then rolls the band boxes (the box of the old position serves the band
+// kernels of the next step, the box of the new position the next fused
+// launch) and, when check_band is set, verifies the centre moved at most
+// kBandDelta per axis.
+__global__ __launch_bounds__(1024) void k_body(Geom g, Ctrl *ctrl,
+                                               const double *__restrict__ partials, int32_t *plist,
+                                               int *pcount, int plist_cap,
+                                               const int32_t *__restrict__ dead, int dead_cap,
+                                               const float *__restrict__ phi,
+                                               const float *__restrict__ ux,
+                                               const float *__restrict__ uy,
+                                               const float *__restrict__ uz, double rhoG,
+                                               double dRho, double gravity, int check_band,
+                                               int reset_lists) {
+  __shared__ int32_t sh[kSortLds];
+  __shared__ double red[1024];
+  const int tid = threadIdx.x;
+  int count = *pcount;
+  if (count > plist_cap) count = plist_cap;
+  int P = 1;
+  while (P < count) P <<= 1;
+  int32_t *w = plist;
+  if (count > 1) {
+    if (P <= kSortLds) {
+      for (int t = tid; t < P; t += blockDim.x) sh[t] = t < count ? plist[t] : INT32_MAX;
+      w = sh;
+    } else {
+      for (int t = count + tid; t < P; t += blockDim.x) plist[t] = INT32_MAX;
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int t = tid; t < P; t += blockDim.x) {
+          const int u = t ^ j;
+          if (u > t) {
+            const int32_t at = w[t], au = w[u];
+            const bool up = (t & k) == 0;
+            if ((at > au) == up) {
+              w[t] = au;
+              w[u] = at;
+            }
+          }
+        }
+        __threadfence_block();
+        __syncthreads();
+      }
+    }
+  } else if (count == 1) {
+    if (tid == 0) sh[0] = plist[0];
+    w = sh;
+    __syncthreads();
+  }
+  const lbm_sphere b0 = ctrl->sphere;
+  // Link momentum: thread t sums sorted entries [t*C, (t+1)*C) in order.
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  {
+    const int C = (count + 1023) / 1024;
+    const int lo = tid * C, hi = min(count, lo + C);
+    for (int c = lo; c < hi; c++) {
+      const int64_t blk = w[c];
+#pragma unroll
+      for (int q = 0; q < 6; q++) acc[q] += partials[blk * 6 + q];
+    }
+  }
+  double link[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) link[q] = block_tree_sum(acc[q], red);
+  // Dead-cell impulses, applyImpulseAtCell at the cell centre.
+  int nd = ctrl->n_dead;
+  if (nd > dead_cap) nd = dead_cap;
+  double dacc[6] = {0, 0, 0, 0, 0, 0};
+  {
+    const int C = (nd + 1023) / 1024;
+    const int lo = tid * C, hi = min(nd, lo + C);
+    for (int c = lo; c < hi; c++) {
+      const int32_t i = dead[c];
+      const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / g.nxny);
+      const double rhoPhase = rhoG + (double)phi[i] * dRho;
+      const double px = rhoPhase * (double)ux[i];
+      const double py = rhoPhase * (double)uy[i];
+      const double pz = rhoPhase * (double)uz[i];
+      const double rx = x + 0.5 - b0.cx;
+      const double ry = y + 0.5 - b0.cy;
+      const double rz = z + 0.5 - b0.cz;
+      dacc[0] += px;
+      dacc[1] += py;
+      dacc[2] += pz;
+      dacc[3] += ry * pz - rz * py;
+      dacc[4] += rz * px - rx * pz;
+      dacc[5] += rx * py - ry * px;
+    }
+  }
+  double dsum[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) dsum[q] = block_tree_sum(dacc[q], red);
+  if (tid != 0) return;
+  lbm_sphere b = b0;
+  b.Fx += link[0];
+  b.Fy += link[1];
+  b.Fz += link[2];
+  b.Tx += link[3];
+  b.Ty += link[4];
+  b.Tz += link[5];
+#pragma unroll
+  for (int q = 0; q < 6; q++) ctrl->F_links[q] = link[q];
+  b.Fx += dsum[0];
+  b.Fy += dsum[1];
+  b.Fz += dsum[2];
+  b.Tx += dsum[3];
+  b.Ty += dsum[4];
+  b.Tz += dsum[5];
+  sphere_step(b, 1.0, gravity);
+  if (check_band) {
+    const double d0 = b.cx - b0.cx, d1 = b.cy - b0.cy, d2 = b.cz - b0.cz;
+    if (!(d0 <= kBandDelta && d0 >= -kBandDelta && d1 <= kBandDelta && d1 >= -kBandDelta &&
+          d2 <= kBandDelta && d2 >= -kBandDelta))
+      ctrl->err |= kErrBandEscape;
+  }
+  ctrl->sphere = b;
+  for (int j = 0; j < 6; j++) ctrl->box_band[j] = ctrl->box_fused[j];
+  int nb[6];
+  band_box(b, g.nx, g.ny, g.nz, nb);
+  for (int j = 0; j < 6; j++) ctrl->box_fused[j] = nb[j];
+  if (reset_lists) {
+    ctrl->n_fresh = 0;
+    ctrl->n_dead = 0;
+    ctrl->n_solid = 0;
+  }
+  *pcount = 0;
+}
+
+}  // namespace lbm

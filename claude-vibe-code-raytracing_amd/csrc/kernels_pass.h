@@ -1,0 +1,575 @@
+// Per-pass kernels: one kernel per pass of Sim.step with the reference's array
+// semantics (f -> f2 -> f, h -> h2 -> h, stored gradients).  They back the
+// per-pass C entry points (the drop-in for code that calls the passes one by
+// one) and the generic-solid path.  Every cell's arithmetic is the reference's:
+// IEEE double in the reference's operation order, float32 stores.
+//
+// Launch shape: blocks of 64x4 cells (one wave per x-row segment), grid
+// (ceil(nx/64), ceil(ny/4), nz).
+#pragma once
+
+#include "device_common.h"
+
+namespace lbm {
+
+#define LBM_CELL_XYZ                                 \
+  const int x = blockIdx.x * 64 + threadIdx.x;       \
+  const int y = blockIdx.y * 4 + threadIdx.y;        \
+  const int z = blockIdx.z;                          \
+  if (x >= g.nx || y >= g.ny) return;
+
+// initFlat body (src/sim.js:112-132): phi from the per-column tanh profile
+// (phicol holds the double phiCol, per y when per_cell == 0, per cell otherwise),
+// rho = 1, u = 0, walls on box faces, f = w, h = w7 * phiCol.
+__global__ __launch_bounds__(256) void k_init_cells(Geom g, const double *__restrict__ phicol,
+                                                    int per_cell, float *__restrict__ phi,
+                                                    float *__restrict__ phiPrev, float *rho,
+                                                    float *ux, float *uy, float *uz, uint8_t *tag,
+                                                    uint8_t *tagPrev, float *f, float *h) {
+  LBM_CELL_XYZ
+  const int64_t i = cell_index(g, x, y, z);
+  const double pc = per_cell ? phicol[i] : phicol[y];
+  phi[i] = (float)pc;
+  phiPrev[i] = (float)pc;
+  rho[i] = 1.0f;
+  ux[i] = 0.0f;
+  uy[i] = 0.0f;
+  uz[i] = 0.0f;
+  const bool onWall = x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1;
+  tag[i] = onWall ? TAG_WALL : TAG_FLUID;
+  tagPrev[i] = tag[i];
+#pragma unroll
+  for (int k = 0; k < Q27; k++) f[k * g.S + i] = w27(k);
+#pragma unroll
+  for (int k = 0; k < Q7; k++) h[k * g.S + i] = (float)((double)w7(k) * pc);
+}
+
+// seedHydrostatic (src/sim.js:144-173): one thread per (x,z) column, sequential
+// prefix along +y.
+__global__ __launch_bounds__(256) void k_seed_hydrostatic(Geom g, double rhoG, double dRho,
+                                                          double gravity,
+                                                          const float *__restrict__ phi,
+                                                          float *__restrict__ rho,
+                                                          float *__restrict__ f) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int z = blockIdx.y;
+  if (x >= g.nx) return;
+  const double gOverCs2 = gravity * INV_CS2_27;
+  const double uLatY = -0.5 * gravity;
+  const double uLat2 = uLatY * uLatY;
+  double pCs2 = 1.0;
+  for (int y = 0; y < g.ny; y++) {
+    const int64_t i = cell_index(g, x, y, z);
+    rho[i] = (float)pCs2;
+    const double rhoPhase = rhoG + (double)phi[i] * dRho;
+#pragma unroll
+    for (int k = 0; k < Q27; k++) {
+      const double eu = (double)EY27[k] * uLatY;
+      const double Qi = eu * (INV_CS2_27 + INV_2CS4_27 * eu) - 1.5 * uLat2;
+      f[k * g.S + i] = (float)((double)w27(k) * (pCs2 + rhoPhase * Qi));
+    }
+    pCs2 += rhoPhase * gOverCs2;
+  }
+}
+
+// Appends index i to a device list; overflow sets kErrListOverflow.
+__device__ __forceinline__ void list_push(int *counter, int32_t *list, int cap, int32_t i,
+                                          int *err) {
+  const int slot = atomicAdd(counter, 1);
+  if (slot < cap) list[slot] = i;
+  else atomicOr(err, kErrListOverflow);
+}
+
+// retagCells (src/sim.js:214-244) over a box of cells [x0,x1]x[y0,y1]x[z0,z1].
+// mode 0: no solid; 1: the device sphere; 2: host mask + surface velocities.
+// copy_prev: also snapshot phi into phiPrev (the full-grid pass does; the fused
+// step's band retag does not, because there phiPrev is the other phi buffer).
+// Cells that turn fluid (fresh) or solid (dead) are appended to the lists, and
+// solid cells are counted.
+__global__ __launch_bounds__(256) void k_retag(Geom g, int mode, const lbm_sphere *__restrict__ sph,
+                                               const uint8_t *__restrict__ mask,
+                                               const float *__restrict__ mus, uint8_t *tag,
+                                               uint8_t *tagPrev, const float *__restrict__ phi,
+                                               float *__restrict__ phiPrev, int copy_prev,
+                                               float *usx, float *usy, float *usz, Ctrl *ctrl,
+                                               int32_t *fresh, int32_t *dead, int cap, int bx0,
+                                               int bx1, int by0, int by1, int bz0, int bz1,
+                                               int use_ctrl_box) {
+  int box[6] = {bx0, bx1, by0, by1, bz0, bz1};
+  if (use_ctrl_box) {
+#pragma unroll
+    for (int j = 0; j < 6; j++) box[j] = ctrl->box_band[j];
+  }
+  const int x = box[0] + blockIdx.x * 64 + threadIdx.x;
+  const int y = box[2] + blockIdx.y * 4 + threadIdx.y;
+  const int z = box[4] + blockIdx.z;
+  if (x > box[1] || y > box[3] || z > box[5]) return;
+  const int64_t i = cell_index(g, x, y, z);
+  const uint8_t tp = tag[i];
+  tagPrev[i] = tp;
+  if (copy_prev) phiPrev[i] = phi[i];
+  uint8_t t;
+  float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+  if (!interior(g, x, y, z)) {
+    t = TAG_WALL;
+  } else {
+    bool solid = false;
+    if (mode == 1) {
+      const lbm_sphere b = *sph;
+      solid = sphere_contains(b, x, y, z);
+      if (solid) {
+        double tmp[3];
+        sphere_surface_velocity(b, x, y, z, tmp);
+        vx = (float)tmp[0];
+        vy = (float)tmp[1];
+        vz = (float)tmp[2];
+      }
+    } else if (mode == 2) {
+      solid = mask[i] != 0;
+      if (solid) {
+        vx = mus[3 * i + 0];
+        vy = mus[3 * i + 1];
+        vz = mus[3 * i + 2];
+      }
+    }
+    t = solid ? TAG_SOLID : TAG_FLUID;
+    if (t == TAG_FLUID && tp == TAG_SOLID) list_push(&ctrl->n_fresh, fresh, cap, (int32_t)i, &ctrl->err);
+    if (t == TAG_SOLID && tp == TAG_FLUID) list_push(&ctrl->n_dead, dead, cap, (int32_t)i, &ctrl->err);
+    if (t == TAG_SOLID) atomicAdd(&ctrl->n_solid, 1);
+  }
+  tag[i] = t;
+  usx[i] = vx;
+  usy[i] = vy;
+  usz[i] = vz;
+}
+
+// Sorts one list ascending in a single workgroup (bitonic).  blockIdx.x selects
+// the fresh (0) or dead (1) list.  Lists up to kSortLds entries sort in LDS;
+// longer ones sort in place in global memory, padded with INT_MAX up to the next
+// power of two (the list capacity is a power of two).
+constexpr int kSortLds = 8192;
+__global__ __launch_bounds__(1024) void k_sort_lists(int32_t *fresh, int32_t *dead, Ctrl *ctrl,
+                                                     int cap) {
+  __shared__ int32_t sh[kSortLds];
+  int32_t *a = blockIdx.x == 0 ? fresh : dead;
+  int count = blockIdx.x == 0 ? ctrl->n_fresh : ctrl->n_dead;
+  if (count > cap) count = cap;
+  if (count <= 1) return;
+  int P = 1;
+  while (P < count) P <<= 1;
+  const int tid = threadIdx.x;
+  int32_t *w;
+  if (P <= kSortLds) {
+    for (int t = tid; t < P; t += blockDim.x) sh[t] = t < count ? a[t] : INT32_MAX;
+    w = sh;
+  } else {
+    for (int t = count + tid; t < P; t += blockDim.x) a[t] = INT32_MAX;
+    w = a;
+  }
+  __threadfence_block();
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = tid; t < P; t += blockDim.x) {
+        const int u = t ^ j;
+        if (u > t) {
+          const int32_t at = w[t], au = w[u];
+          const bool up = (t & k) == 0;
+          if ((at > au) == up) {
+            w[t] = au;
+            w[u] = at;
+          }
+        }
+      }
+      __threadfence_block();
+      __syncthreads();
+    }
+  }
+  if (P <= kSortLds)
+    for (int t = tid; t < count; t += blockDim.x) a[t] = sh[t];
+}
+
+__device__ __forceinline__ int lower_bound_i32(const int32_t *a, int n, int32_t v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// handleFreshCells (src/sim.js:257-310) over the sorted fresh list, in one
+// workgroup.  The reference updates cells in place in index order, so a fresh
+// cell averages the already-reseeded values of earlier fresh neighbours and the
+// stale values of later ones.  Cells are scheduled by dependency level: a cell's
+// level is one more than the highest level of its earlier fresh neighbours, and
+// all cells of a level are processed together, which reproduces the sequential
+// order exactly (a later fresh neighbour always has a higher level).
+//
+// phi_nb: phi read for neighbours; phi_fb: phiPrev read for the no-neighbour
+// fallback; phi_w1/phi_w2: the two writes of phi[i] = phiPrev[i] = p (they may
+// alias).  f and h receive the reseeded populations.
+__global__ __launch_bounds__(1024) void k_fresh(Geom g, double rhoG, double rhoL,
+                                                const int32_t *__restrict__ list, Ctrl *ctrl,
+                                                int cap, int *level_scratch,
+                                                const uint8_t *__restrict__ tag,
+                                                const float *phi_fb, float *phi_w1, float *phi_w2,
+                                                float *rho, float *ux, float *uy, float *uz,
+                                                float *f, float *h) {
+  __shared__ int sh_level[4096];
+  __shared__ int sh_flag;
+  int count = ctrl->n_fresh;
+  if (count > cap) count = cap;
+  if (count == 0) return;
+  const int tid = threadIdx.x;
+  int *lv = count <= 4096 ? sh_level : level_scratch;
+  for (int c = tid; c < count; c += blockDim.x) lv[c] = 0;
+  __threadfence_block();
+  __syncthreads();
+  // Longest-path levels by repeated relaxation (terminates after depth+1 rounds).
+  for (;;) {
+    if (tid == 0) sh_flag = 0;
+    __syncthreads();
+    for (int c = tid; c < count; c += blockDim.x) {
+      const int32_t i = list[c];
+      const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / g.nxny);
+      int L = 0;
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+          for (int dx = -1; dx <= 1; dx++) {
+            if (!dx && !dy && !dz) continue;
+            const int64_t j = cell_index(g, x + dx, y + dy, z + dz);
+            if (j >= i || tag[j] != TAG_FLUID) continue;
+            const int q = lower_bound_i32(list, c, (int32_t)j);
+            if (q < c && list[q] == (int32_t)j) {
+              const int lq = lv[q] + 1;
+              if (lq > L) L = lq;
+            }
+          }
+      if (L != lv[c]) {
+        lv[c] = L;
+        sh_flag = 1;
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (!sh_flag) break;
+    __syncthreads();
+  }
+  int maxL = 0;
+  for (int c = 0; c < count; c++) maxL = lv[c] > maxL ? lv[c] : maxL;
+  const double dRho = rhoL - rhoG;
+  for (int L = 0; L <= maxL; L++) {
+    for (int c = tid; c < count; c += blockDim.x) {
+      if (lv[c] != L) continue;
+      const int32_t i = list[c];
+      const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / g.nxny);
+      double psum = 0, rsum = 0, vxs = 0, vys = 0, vzs = 0;
+      int pcnt = 0;
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+          for (int dx = -1; dx <= 1; dx++) {
+            if (!dx && !dy && !dz) continue;
+            const int64_t j = cell_index(g, x + dx, y + dy, z + dz);
+            if (tag[j] == TAG_FLUID) {
+              psum += (double)phi_w1[j];
+              rsum += (double)rho[j];
+              vxs += (double)ux[j];
+              vys += (double)uy[j];
+              vzs += (double)uz[j];
+              pcnt++;
+            }
+          }
+      const double p = pcnt > 0 ? psum / pcnt : (double)phi_fb[i];
+      const double r = pcnt > 0 ? rsum / pcnt : 1.0;
+      const double vx = pcnt > 0 ? vxs / pcnt : 0;
+      const double vy = pcnt > 0 ? vys / pcnt : 0;
+      const double vz = pcnt > 0 ? vzs / pcnt : 0;
+      phi_w1[i] = (float)p;
+      phi_w2[i] = (float)p;
+      rho[i] = (float)r;
+      ux[i] = (float)vx;
+      uy[i] = (float)vy;
+      uz[i] = (float)vz;
+      const double rhoPhase = rhoG + p * dRho;
+      const double u2 = vx * vx + vy * vy + vz * vz;
+#pragma unroll
+      for (int k = 0; k < Q27; k++) {
+        const double eu = EX27[k] * vx + EY27[k] * vy + EZ27[k] * vz;
+        const double Qi = eu * (INV_CS2_27 + INV_2CS4_27 * eu) - 1.5 * u2;
+        f[k * g.S + i] = (float)((double)w27(k) * (r + rhoPhase * Qi));
+      }
+#pragma unroll
+      for (int k = 0; k < Q7; k++) {
+        const double eu = EX7[k] * vx + EY7[k] * vy + EZ7[k] * vz;
+        h[k * g.S + i] = (float)((double)w7(k) * p * (1 + INV_CS2_7 * eu));
+      }
+    }
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+// computePhiGradients (src/sim.js:176-199): all cells, clamped indices.
+__global__ __launch_bounds__(256) void k_gradients(Geom g, const float *__restrict__ phi,
+                                                   float *__restrict__ gx, float *__restrict__ gy,
+                                                   float *__restrict__ gz,
+                                                   float *__restrict__ lap) {
+  LBM_CELL_XYZ
+  const int xm = x > 0 ? x - 1 : x, xp = x < g.nx - 1 ? x + 1 : x;
+  const int ym = y > 0 ? y - 1 : y, yp = y < g.ny - 1 ? y + 1 : y;
+  const int zm = z > 0 ? z - 1 : z, zp = z < g.nz - 1 ? z + 1 : z;
+  const int64_t i = cell_index(g, x, y, z);
+  const double c = phi[i];
+  const double px = phi[cell_index(g, xp, y, z)];
+  const double mx = phi[cell_index(g, xm, y, z)];
+  const double py = phi[cell_index(g, x, yp, z)];
+  const double my = phi[cell_index(g, x, ym, z)];
+  const double pz = phi[cell_index(g, x, y, zp)];
+  const double mz = phi[cell_index(g, x, y, zm)];
+  gx[i] = (float)(0.5 * (px - mx));
+  gy[i] = (float)(0.5 * (py - my));
+  gz[i] = (float)(0.5 * (pz - mz));
+  lap[i] = (float)(px + mx + py + my + pz + mz - 6 * c);
+}
+
+// collidePhase (src/sim.js:324-364).
+__global__ __launch_bounds__(256) void k_collide_phase(Geom g, Derived d,
+                                                       const float *__restrict__ h,
+                                                       float *__restrict__ h2,
+                                                       const float *__restrict__ phi,
+                                                       const float *__restrict__ ux,
+                                                       const float *__restrict__ uy,
+                                                       const float *__restrict__ uz,
+                                                       const uint8_t *__restrict__ tag,
+                                                       const float *__restrict__ gx,
+                                                       const float *__restrict__ gy,
+                                                       const float *__restrict__ gz) {
+  LBM_CELL_XYZ
+  if (!interior(g, x, y, z)) return;
+  const int64_t i = cell_index(g, x, y, z);
+  if (tag[i] != TAG_FLUID) return;
+  const double p = phi[i];
+  const double uxi = ux[i], uyi = uy[i], uzi = uz[i];
+  const double ggx = gx[i], ggy = gy[i], ggz = gz[i];
+  const double gmag = sqrt(ggx * ggx + ggy * ggy + ggz * ggz) + 1e-12;
+  const double sharpen = d.sharpC * p * (1 - p);
+  const double Sx = sharpen * d.sharpFac * ggx / gmag;
+  const double Sy = sharpen * d.sharpFac * ggy / gmag;
+  const double Sz = sharpen * d.sharpFac * ggz / gmag;
+#pragma unroll
+  for (int k = 0; k < Q7; k++) {
+    const double ekx = EX7[k], eky = EY7[k], ekz = EZ7[k];
+    const double wk = w7(k);
+    const double eu = ekx * uxi + eky * uyi + ekz * uzi;
+    const double heq = wk * p * (1 + INV_CS2_7 * eu);
+    const double Fk = wk * (ekx * Sx + eky * Sy + ekz * Sz);
+    const double hk = h[k * g.S + i];
+    h2[k * g.S + i] = (float)(hk - d.invTauPhi * (hk - heq) + Fk);
+  }
+}
+
+// streamPhase (src/sim.js:376-397): pull; a non-fluid source bounces back.
+__global__ __launch_bounds__(256) void k_stream_phase(Geom g, const float *__restrict__ h2,
+                                                      float *__restrict__ h,
+                                                      const uint8_t *__restrict__ tag) {
+  LBM_CELL_XYZ
+  if (!interior(g, x, y, z)) return;
+  const int64_t i = cell_index(g, x, y, z);
+  if (tag[i] != TAG_FLUID) return;
+#pragma unroll
+  for (int k = 0; k < Q7; k++) {
+    const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
+    h[k * g.S + i] = tag[s] == TAG_FLUID ? h2[k * g.S + s] : h2[opp7(k) * g.S + i];
+  }
+}
+
+// collideHydro (src/sim.js:406-464).
+__global__ __launch_bounds__(256) void k_collide_hydro(
+    Geom g, Derived d, const float *__restrict__ f, float *__restrict__ f2,
+    const float *__restrict__ rho, const float *__restrict__ ux, const float *__restrict__ uy,
+    const float *__restrict__ uz, const float *__restrict__ phi, const uint8_t *__restrict__ tag,
+    const float *__restrict__ gx, const float *__restrict__ gy, const float *__restrict__ gz,
+    const float *__restrict__ lap, const float *__restrict__ bfx, const float *__restrict__ bfy,
+    const float *__restrict__ bfz, int use_bf) {
+  LBM_CELL_XYZ
+  if (!interior(g, x, y, z)) return;
+  const int64_t i = cell_index(g, x, y, z);
+  if (tag[i] != TAG_FLUID) return;
+  const double sumF = rho[i];
+  const double p = phi[i];
+  const double rhoPhase = d.rhoG + p * d.dRho;
+  const double mu = 4 * d.beta * p * (p - 1) * (2 * p - 1) - d.kappa * (double)lap[i];
+  const double bx = use_bf ? (double)bfx[i] : 0.0;
+  const double by = use_bf ? (double)bfy[i] : 0.0;
+  const double bz = use_bf ? (double)bfz[i] : 0.0;
+  const double fx = bx + mu * (double)gx[i];
+  const double fy = by + mu * (double)gy[i] + rhoPhase * d.gravity;
+  const double fz = bz + mu * (double)gz[i];
+  const double uxe = ux[i], uye = uy[i], uze = uz[i];
+  const double u2 = uxe * uxe + uye * uye + uze * uze;
+  const double negU2half = -1.5 * u2;
+  const double uF = uxe * fx + uye * fy + uze * fz;
+#pragma unroll
+  for (int k = 0; k < Q27; k++) {
+    const double ekx = EX27[k], eky = EY27[k], ekz = EZ27[k];
+    const double wk = w27(k);
+    const double eu = ekx * uxe + eky * uye + ekz * uze;
+    const double Qi = negU2half + eu * (INV_CS2_27 + INV_2CS4_27 * eu);
+    const double feq = wk * (sumF + rhoPhase * Qi);
+    const double ekF = ekx * fx + eky * fy + ekz * fz;
+    const double Sk = d.A * wk * (ekF * (1 + INV_CS2_27 * eu) - uF);
+    const double fk = f[k * g.S + i];
+    f2[k * g.S + i] = (float)(fk - d.invTau * (fk - feq) + Sk);
+  }
+}
+
+// streamHydroAndMacro (src/sim.js:484-582).  With emit_links, each fluid->solid
+// link appends (key = i*27 + k, m = f_opp + f_k) so the host can replay the
+// applyImpulseAtCell calls in the reference's order.
+__global__ __launch_bounds__(256) void k_stream_hydro_macro(
+    Geom g, Derived d, const float *__restrict__ f2, float *__restrict__ f,
+    const float *__restrict__ h, float *__restrict__ rho, float *__restrict__ ux,
+    float *__restrict__ uy, float *__restrict__ uz, float *__restrict__ phi,
+    const uint8_t *__restrict__ tag, const float *__restrict__ usx, const float *__restrict__ usy,
+    const float *__restrict__ usz, const float *__restrict__ gx, const float *__restrict__ gy,
+    const float *__restrict__ gz, const float *__restrict__ lap, const float *__restrict__ bfx,
+    const float *__restrict__ bfy, const float *__restrict__ bfz, int use_bf, int emit_links,
+    int64_t *link_key, double *link_m, int link_cap, Ctrl *ctrl) {
+  LBM_CELL_XYZ
+  if (!interior(g, x, y, z)) return;
+  const int64_t i = cell_index(g, x, y, z);
+  if (tag[i] != TAG_FLUID) return;
+  const double rhoPhaseOld = d.rhoG + (double)phi[i] * d.dRho;
+  double pSumF = 0, mx = 0, my = 0, mz = 0;
+#pragma unroll
+  for (int k = 0; k < Q27; k++) {
+    const int64_t s = i - ((int64_t)EZ27[k] * g.nxny + (int64_t)EY27[k] * g.nx + EX27[k]);
+    const uint8_t st = tag[s];
+    const double ekx = EX27[k], eky = EY27[k], ekz = EZ27[k];
+    double fk;
+    if (st == TAG_FLUID) {
+      fk = f2[k * g.S + s];
+    } else {
+      const double fopp = f2[opp27(k) * g.S + i];
+      double corr = 0;
+      if (st == TAG_SOLID) {
+        const double eu = ekx * (double)usx[s] + eky * (double)usy[s] + ekz * (double)usz[s];
+        corr = 2 * (double)w27(k) * rhoPhaseOld * eu * INV_CS2_27;
+      }
+      fk = fopp + corr;
+      if (st == TAG_SOLID && emit_links) {
+        const double m = fopp + fk;
+        const int slot = atomicAdd(&ctrl->n_links, 1);
+        if (slot < link_cap) {
+          link_key[slot] = i * Q27 + k;
+          link_m[slot] = m;
+        } else {
+          atomicOr(&ctrl->err, kErrListOverflow);
+        }
+      }
+    }
+    f[k * g.S + i] = (float)fk;
+    pSumF += fk;
+    mx += ekx * fk;
+    my += eky * fk;
+    mz += ekz * fk;
+  }
+  rho[i] = (float)(pSumF > 0.01 ? pSumF : 0.01);
+  double phiSum = 0;
+#pragma unroll
+  for (int k = 0; k < Q7; k++) phiSum += (double)h[k * g.S + i];
+  const double p = phiSum < 0 ? 0 : phiSum > 1 ? 1 : phiSum;
+  phi[i] = (float)p;
+  const double rhoPhaseNew = d.rhoG + p * d.dRho;
+  const double mu = 4 * d.beta * p * (p - 1) * (2 * p - 1) - d.kappa * (double)lap[i];
+  const double bx = use_bf ? (double)bfx[i] : 0.0;
+  const double by = use_bf ? (double)bfy[i] : 0.0;
+  const double bz = use_bf ? (double)bfz[i] : 0.0;
+  const double fx = bx + mu * (double)gx[i];
+  const double fy = by + mu * (double)gy[i] + rhoPhaseNew * d.gravity;
+  const double fz = bz + mu * (double)gz[i];
+  const double invRho = 1 / rhoPhaseNew;
+  double uxn = (mx + 0.5 * fx) * invRho;
+  double uyn = (my + 0.5 * fy) * invRho;
+  double uzn = (mz + 0.5 * fz) * invRho;
+  const double u2 = uxn * uxn + uyn * uyn + uzn * uzn;
+  const double UMAX = 0.2, UMAX2 = UMAX * UMAX;
+  if (u2 > UMAX2) {
+    const double sc = UMAX / sqrt(u2);
+    uxn *= sc;
+    uyn *= sc;
+    uzn *= sc;
+  }
+  ux[i] = (float)uxn;
+  uy[i] = (float)uyn;
+  uz[i] = (float)uzn;
+}
+
+// Counts solid cells into ctrl->n_solid (bounds the momentum-exchange link list:
+// every link has a solid source cell, and each cell sources at most 26 links).
+__global__ __launch_bounds__(256) void k_count_solid(int64_t n, const uint8_t *__restrict__ tag,
+                                                     Ctrl *ctrl) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int local = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    local += tag[i] == TAG_SOLID;
+  if (local) atomicAdd(&cnt, local);
+  __syncthreads();
+  if (threadIdx.x == 0 && cnt) atomicAdd(&ctrl->n_solid, cnt);
+}
+
+// Dead-cell impulses rho_phase * u (src/solid.js:94-97) for the sorted dead list.
+__global__ __launch_bounds__(256) void k_dead_impulses(const int32_t *__restrict__ dead,
+                                                       const Ctrl *ctrl, int cap, double rhoG,
+                                                       double dRho, const float *__restrict__ phi,
+                                                       const float *__restrict__ ux,
+                                                       const float *__restrict__ uy,
+                                                       const float *__restrict__ uz,
+                                                       double *__restrict__ imp) {
+  int count = ctrl->n_dead;
+  if (count > cap) count = cap;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < count; c += gridDim.x * blockDim.x) {
+    const int32_t i = dead[c];
+    const double rhoPhase = rhoG + (double)phi[i] * dRho;
+    imp[3 * c + 0] = rhoPhase * (double)ux[i];
+    imp[3 * c + 1] = rhoPhase * (double)uy[i];
+    imp[3 * c + 2] = rhoPhase * (double)uz[i];
+  }
+}
+
+// packVolume (src/sim.js:586-600).
+__global__ __launch_bounds__(256) void k_pack(int64_t n, const float *__restrict__ phi,
+                                              const uint8_t *__restrict__ tag,
+                                              uint8_t *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double p = phi[i];
+  if (p < 0) p = 0;
+  else if (p > 1) p = 1;
+  out[2 * i] = (uint8_t)(int32_t)(p * 255);
+  out[2 * i + 1] = tag[i] == TAG_SOLID ? 255 : 0;
+}
+
+// Layout transposes between the reference's array-of-structures (q values per
+// cell) and the device planes.
+__global__ __launch_bounds__(256) void k_aos_to_soa(int64_t n, int64_t S, int q,
+                                                    const float *__restrict__ aos,
+                                                    float *__restrict__ soa) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int k = 0; k < q; k++) soa[k * S + i] = aos[i * q + k];
+}
+__global__ __launch_bounds__(256) void k_soa_to_aos(int64_t n, int64_t S, int q,
+                                                    const float *__restrict__ soa,
+                                                    float *__restrict__ aos) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int k = 0; k < q; k++) aos[i * q + k] = soa[k * S + i];
+}
+
+}  // namespace lbm

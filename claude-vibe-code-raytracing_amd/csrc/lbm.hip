@@ -1,0 +1,881 @@
+// liblbm: C ABI of the coupled LBM step on MI355X (declarations in include/lbm.h).
+//
+// Host side of the library: device state, the per-pass entry points (the
+// reference's Sim passes one by one), the fused coupled step, field transfer and
+// the host RigidSphere functions.  Kernels live in kernels_*.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "lbm.h"
+#include "device_common.h"
+#include "kernels_pass.h"
+#include "kernels_fused.h"
+
+using namespace lbm;
+
+struct lbm_sim {
+  int nx, ny, nz;
+  Geom g;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  lbm_params params{};
+  // Population buffers: fbuf[fcur] is the reference's f, fbuf[1 - fcur] its f2;
+  // likewise for h.  phibuf[phicur] is phi, phibuf[1 - phicur] phiPrev.
+  float *fbuf[2] = {nullptr, nullptr};
+  float *hbuf[2] = {nullptr, nullptr};
+  int fcur = 0, hcur = 0;
+  float *phibuf[2] = {nullptr, nullptr};
+  int phicur = 0;
+  float *rho = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr;
+  uint8_t *tag = nullptr, *tagPrev = nullptr;
+  float *usx = nullptr, *usy = nullptr, *usz = nullptr;
+  float *gx = nullptr, *gy = nullptr, *gz = nullptr, *lap = nullptr;
+  float *bfx = nullptr, *bfy = nullptr, *bfz = nullptr;
+  bool bodyF_armed = false;
+  // Gradients stored in gx..lap are those of the last computePhiGradients; after
+  // a fused step they are recomputed from phiPrev when read (grads_stale).
+  bool grads_stale = false;
+  // Lists of the last retag.
+  int32_t *fresh = nullptr, *dead = nullptr;
+  int list_cap = 0;  // power of two >= n
+  int *level_scratch = nullptr;
+  Ctrl *ctrl = nullptr;       // device
+  Ctrl *ctrl_host = nullptr;  // pinned mirror
+  // Momentum-exchange link list (per-pass path).
+  int64_t *link_key = nullptr;
+  double *link_m = nullptr;
+  int link_cap = 0;
+  double *imp = nullptr;  // dead-cell impulses
+  int imp_cap = 0;
+  // Staging for host transfers and layout transposes.
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  uint8_t *mask_dev = nullptr;
+  float *mus_dev = nullptr;
+  // Fused step.
+  double *partials = nullptr;  // per-block link momentum, 6 doubles per block
+  int64_t partial_blocks = 0;
+  bool need_full_retag = true;  // next coupled step must retag the whole grid
+  bool sphere_dev_valid = false;
+  std::vector<lbm_link> links;  // kept by lbm_stream_hydro_macro_links
+  // Timing.
+  bool timing = false;
+  std::map<std::string, std::pair<double, int64_t>> ktime;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::string err;
+};
+
+static std::string g_create_error;
+
+// ------------------------------------------------------------------ helpers
+
+#define HIP_TRY(sim, expr)                                                              \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      (sim)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                  \
+      return LBM_ERR_HIP;                                                               \
+    }                                                                                   \
+  } while (0)
+
+#define CHECK_SIM(sim)                    \
+  do {                                    \
+    if (!(sim)) return LBM_ERR_ARG;       \
+  } while (0)
+
+static int fail(lbm_sim *s, int code, const std::string &msg) {
+  s->err = msg;
+  return code;
+}
+
+static Derived derive(const lbm_params &p) {
+  // src/sim.js:327-332 and :410-416, :490-492, evaluated exactly as there.
+  Derived d;
+  d.invTau = 1 / p.tau;
+  d.beta = 12 * p.sigma / p.W;
+  d.kappa = 1.5 * p.sigma * p.W;
+  d.dRho = p.rhoL - p.rhoG;
+  d.A = (1 - 0.5 * d.invTau) * INV_CS2_27;
+  d.rhoG = p.rhoG;
+  d.gravity = p.gravity;
+  d.invTauPhi = 1 / p.tauPhi;
+  const double Mphi = CS2_7 * (p.tauPhi - 0.5);
+  const double guoHalf = 1.0 - 0.5 * d.invTauPhi;
+  d.sharpC = 4.0 * Mphi * guoHalf / p.W;
+  d.sharpFac = INV_CS2_7;
+  return d;
+}
+
+static dim3 cell_grid(const lbm_sim *s) {
+  return dim3((s->nx + 63) / 64, (s->ny + 3) / 4, s->nz);
+}
+static const dim3 kCellBlock(64, 4, 1);
+
+static float *F(lbm_sim *s) { return s->fbuf[s->fcur]; }
+static float *F2(lbm_sim *s) { return s->fbuf[1 - s->fcur]; }
+static float *H(lbm_sim *s) { return s->hbuf[s->hcur]; }
+static float *H2(lbm_sim *s) { return s->hbuf[1 - s->hcur]; }
+static float *PHI(lbm_sim *s) { return s->phibuf[s->phicur]; }
+static float *PHIPREV(lbm_sim *s) { return s->phibuf[1 - s->phicur]; }
+
+static int ensure_scratch(lbm_sim *s, size_t bytes) {
+  if (s->scratch_bytes >= bytes) return LBM_OK;
+  if (s->scratch) hipFree(s->scratch);
+  s->scratch = nullptr;
+  s->scratch_bytes = 0;
+  HIP_TRY(s, hipMalloc(&s->scratch, bytes));
+  s->scratch_bytes = bytes;
+  return LBM_OK;
+}
+
+static int ensure_links(lbm_sim *s, int64_t cap) {
+  if (cap > INT32_MAX) return fail(s, LBM_ERR_CAPACITY, "link list capacity exceeds 2^31");
+  if (s->link_cap >= cap) return LBM_OK;
+  hipFree(s->link_key);
+  hipFree(s->link_m);
+  s->link_key = nullptr;
+  s->link_m = nullptr;
+  s->link_cap = 0;
+  HIP_TRY(s, hipMalloc(&s->link_key, cap * sizeof(int64_t)));
+  HIP_TRY(s, hipMalloc(&s->link_m, cap * sizeof(double)));
+  s->link_cap = (int)cap;
+  return LBM_OK;
+}
+
+static int ensure_imp(lbm_sim *s, int64_t cap) {
+  if (s->imp_cap >= cap) return LBM_OK;
+  hipFree(s->imp);
+  s->imp = nullptr;
+  s->imp_cap = 0;
+  HIP_TRY(s, hipMalloc(&s->imp, std::max<int64_t>(cap, 1) * 3 * sizeof(double)));
+  s->imp_cap = (int)cap;
+  return LBM_OK;
+}
+
+static int sync(lbm_sim *s) {
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  return LBM_OK;
+}
+
+static int read_ctrl(lbm_sim *s) {
+  HIP_TRY(s, hipMemcpyAsync(s->ctrl_host, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
+  return sync(s);
+}
+
+static int check_launch(lbm_sim *s, const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(s, LBM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return LBM_OK;
+}
+
+// Timing brackets around a launch (only when timing is on).
+struct TimedLaunch {
+  lbm_sim *s;
+  const char *name;
+  hipEvent_t a = nullptr, b = nullptr;
+  TimedLaunch(lbm_sim *s_, const char *n) : s(s_), name(n) {
+    if (s->timing) {
+      hipEventCreate(&a);
+      hipEventCreate(&b);
+      hipEventRecord(a, s->stream);
+    }
+  }
+  ~TimedLaunch() {
+    if (s->timing) {
+      hipEventRecord(b, s->stream);
+      s->pending.push_back({name, {a, b}});
+    }
+  }
+};
+
+static void collect_timing(lbm_sim *s) {
+  for (auto &p : s->pending) {
+    float ms = 0;
+    hipEventSynchronize(p.second.second);
+    hipEventElapsedTime(&ms, p.second.first, p.second.second);
+    auto &e = s->ktime[p.first];
+    e.first += ms;
+    e.second += 1;
+    hipEventDestroy(p.second.first);
+    hipEventDestroy(p.second.second);
+  }
+  s->pending.clear();
+}
+
+// ------------------------------------------------------------------ lifetime
+
+extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
+  if (!out) return LBM_ERR_ARG;
+  *out = nullptr;
+  if (nx < 3 || ny < 3 || nz < 3) {
+    g_create_error = "grid must be at least 3 cells on each axis";
+    return LBM_ERR_ARG;
+  }
+  const int64_t n = (int64_t)nx * ny * nz;
+  if (n > ((int64_t)1 << 30)) {
+    g_create_error = "grid too large: at most 2^30 cells";
+    return LBM_ERR_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    g_create_error = "no HIP device available";
+    return LBM_ERR_NODEVICE;
+  }
+  lbm_sim *s = new lbm_sim();
+  s->nx = nx;
+  s->ny = ny;
+  s->nz = nz;
+  s->g.nx = nx;
+  s->g.ny = ny;
+  s->g.nz = nz;
+  s->g.nxny = nx * ny;
+  s->g.n = n;
+  s->g.S = (n + 63) / 64 * 64;
+  hipGetDevice(&s->device);
+  // src/sim.js:88-95 defaults.
+  s->params = {0.6, 0.7, 0.012, 0.0010, 1.0, 0.05, 4.0, 0.02};
+  int cap = 1;
+  while (cap < n) cap <<= 1;
+  s->list_cap = cap;
+  const int64_t S = s->g.S;
+  hipError_t e = hipSuccess;
+#define ALLOC(ptr, bytes)                                        \
+  if (e == hipSuccess) {                                         \
+    e = hipMalloc((void **)&(ptr), (bytes));                     \
+  }
+  ALLOC(s->fbuf[0], S * 27 * sizeof(float));
+  ALLOC(s->fbuf[1], S * 27 * sizeof(float));
+  ALLOC(s->hbuf[0], S * 7 * sizeof(float));
+  ALLOC(s->hbuf[1], S * 7 * sizeof(float));
+  ALLOC(s->phibuf[0], S * sizeof(float));
+  ALLOC(s->phibuf[1], S * sizeof(float));
+  ALLOC(s->rho, S * sizeof(float));
+  ALLOC(s->ux, S * sizeof(float));
+  ALLOC(s->uy, S * sizeof(float));
+  ALLOC(s->uz, S * sizeof(float));
+  ALLOC(s->tag, S);
+  ALLOC(s->tagPrev, S);
+  ALLOC(s->usx, S * sizeof(float));
+  ALLOC(s->usy, S * sizeof(float));
+  ALLOC(s->usz, S * sizeof(float));
+  ALLOC(s->gx, S * sizeof(float));
+  ALLOC(s->gy, S * sizeof(float));
+  ALLOC(s->gz, S * sizeof(float));
+  ALLOC(s->lap, S * sizeof(float));
+  ALLOC(s->bfx, S * sizeof(float));
+  ALLOC(s->bfy, S * sizeof(float));
+  ALLOC(s->bfz, S * sizeof(float));
+  ALLOC(s->fresh, (size_t)cap * sizeof(int32_t));
+  ALLOC(s->dead, (size_t)cap * sizeof(int32_t));
+  ALLOC(s->level_scratch, (size_t)cap * sizeof(int));
+  ALLOC(s->ctrl, sizeof(Ctrl));
+#undef ALLOC
+  if (e == hipSuccess) e = hipHostMalloc((void **)&s->ctrl_host, sizeof(Ctrl), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) {
+    // Zero the whole state so reads before an init are defined.
+    const int64_t fb = S * sizeof(float);
+    for (int b = 0; b < 2; b++) {
+      hipMemsetAsync(s->fbuf[b], 0, fb * 27, s->stream);
+      hipMemsetAsync(s->hbuf[b], 0, fb * 7, s->stream);
+      hipMemsetAsync(s->phibuf[b], 0, fb, s->stream);
+    }
+    float *arrs[] = {s->rho, s->ux, s->uy, s->uz, s->usx, s->usy, s->usz, s->gx, s->gy, s->gz,
+                     s->lap, s->bfx, s->bfy, s->bfz};
+    for (float *a : arrs) hipMemsetAsync(a, 0, fb, s->stream);
+    hipMemsetAsync(s->tag, 0, S, s->stream);
+    hipMemsetAsync(s->tagPrev, 0, S, s->stream);
+    hipMemsetAsync(s->ctrl, 0, sizeof(Ctrl), s->stream);
+    e = hipStreamSynchronize(s->stream);
+  }
+  if (e != hipSuccess) {
+    g_create_error = std::string("allocation of device state failed: ") + hipGetErrorString(e);
+    lbm_destroy(s);
+    return e == hipErrorOutOfMemory ? LBM_ERR_OOM : LBM_ERR_HIP;
+  }
+  *out = s;
+  return LBM_OK;
+}
+
+extern "C" void lbm_destroy(lbm_sim *s) {
+  if (!s) return;
+  if (s->stream) hipStreamSynchronize(s->stream);
+  collect_timing(s);
+  void *ptrs[] = {s->fbuf[0], s->fbuf[1], s->hbuf[0], s->hbuf[1], s->phibuf[0], s->phibuf[1],
+                  s->rho, s->ux, s->uy, s->uz, s->tag, s->tagPrev, s->usx, s->usy, s->usz, s->gx,
+                  s->gy, s->gz, s->lap, s->bfx, s->bfy, s->bfz, s->fresh, s->dead,
+                  s->level_scratch, s->ctrl, s->link_key, s->link_m, s->imp, s->scratch,
+                  s->mask_dev, s->mus_dev, s->partials};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  if (s->ctrl_host) hipHostFree(s->ctrl_host);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+}
+
+extern "C" const char *lbm_last_error(const lbm_sim *s) {
+  return s ? s->err.c_str() : g_create_error.c_str();
+}
+
+extern "C" int lbm_dims(const lbm_sim *s, int *nx, int *ny, int *nz) {
+  CHECK_SIM(s);
+  if (nx) *nx = s->nx;
+  if (ny) *ny = s->ny;
+  if (nz) *nz = s->nz;
+  return LBM_OK;
+}
+
+extern "C" int lbm_set_params(lbm_sim *s, const lbm_params *p) {
+  CHECK_SIM(s);
+  if (!p) return fail(s, LBM_ERR_ARG, "params is NULL");
+  s->params = *p;
+  return LBM_OK;
+}
+
+extern "C" int lbm_get_params(const lbm_sim *s, lbm_params *p) {
+  CHECK_SIM(s);
+  if (!p) return LBM_ERR_ARG;
+  *p = s->params;
+  return LBM_OK;
+}
+
+extern "C" const char *lbm_version(void) { return "liblbm 0.1 gfx950"; }
+
+extern "C" void *lbm_stream(lbm_sim *s) { return s ? (void *)s->stream : nullptr; }
+
+extern "C" int lbm_synchronize(lbm_sim *s) {
+  CHECK_SIM(s);
+  return sync(s);
+}
+
+// ------------------------------------------------------------------ init
+
+static int init_columns(lbm_sim *s, const double *yw, double waterline) {
+  const int nx = s->nx, ny = s->ny, nz = s->nz;
+  const double W = s->params.W;
+  std::vector<double> col;
+  int per_cell = yw != nullptr;
+  if (!per_cell) {
+    col.resize(ny);
+    // src/sim.js:116
+    for (int y = 0; y < ny; y++) col[y] = 0.5 * (1 + std::tanh(2 * (y - waterline) / W));
+  } else {
+    col.resize((size_t)s->g.n);
+    for (int z = 0; z < nz; z++)
+      for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) {
+          const double wl = yw[(size_t)z * nx + x];
+          col[((size_t)z * ny + y) * nx + x] = 0.5 * (1 + std::tanh(2 * (y - wl) / W));
+        }
+  }
+  const size_t bytes = col.size() * sizeof(double);
+  int rc = ensure_scratch(s, bytes);
+  if (rc) return rc;
+  HIP_TRY(s, hipMemcpyAsync(s->scratch, col.data(), bytes, hipMemcpyHostToDevice, s->stream));
+  k_init_cells<<<cell_grid(s), kCellBlock, 0, s->stream>>>(
+      s->g, (const double *)s->scratch, per_cell, PHI(s), PHIPREV(s), s->rho, s->ux, s->uy, s->uz,
+      s->tag, s->tagPrev, F(s), H(s));
+  if ((rc = check_launch(s, "k_init_cells"))) return rc;
+  rc = lbm_seed_hydrostatic(s);
+  if (rc) return rc;
+  s->need_full_retag = true;
+  s->grads_stale = false;
+  return sync(s);
+}
+
+extern "C" int lbm_init_flat(lbm_sim *s, double waterline_y) {
+  CHECK_SIM(s);
+  return init_columns(s, nullptr, waterline_y);
+}
+
+extern "C" int lbm_init_waterline_map(lbm_sim *s, const double *yw) {
+  CHECK_SIM(s);
+  if (!yw) return fail(s, LBM_ERR_ARG, "yw is NULL");
+  return init_columns(s, yw, 0.0);
+}
+
+extern "C" int lbm_seed_hydrostatic(lbm_sim *s) {
+  CHECK_SIM(s);
+  const lbm_params &p = s->params;
+  dim3 grid((s->nx + 63) / 64, s->nz);
+  k_seed_hydrostatic<<<grid, 64, 0, s->stream>>>(s->g, p.rhoG, p.rhoL - p.rhoG, p.gravity,
+                                                 PHI(s), s->rho, F(s));
+  return check_launch(s, "k_seed_hydrostatic");
+}
+
+// ------------------------------------------------------------------ passes
+
+static int retag_full(lbm_sim *s, int mode) {
+  HIP_TRY(s, hipMemsetAsync(s->ctrl, 0, 4 * sizeof(int), s->stream));  // list counters
+  k_retag<<<cell_grid(s), kCellBlock, 0, s->stream>>>(
+      s->g, mode, &s->ctrl->sphere, s->mask_dev, s->mus_dev, s->tag, s->tagPrev, PHI(s),
+      PHIPREV(s), 1, s->usx, s->usy, s->usz, s->ctrl, s->fresh, s->dead, s->list_cap, 0,
+      s->nx - 1, 0, s->ny - 1, 0, s->nz - 1, 0);
+  int rc = check_launch(s, "k_retag");
+  if (rc) return rc;
+  k_sort_lists<<<2, 1024, 0, s->stream>>>(s->fresh, s->dead, s->ctrl, s->list_cap);
+  return check_launch(s, "k_sort_lists");
+}
+
+static int upload_sphere(lbm_sim *s, const lbm_sphere *b) {
+  HIP_TRY(s, hipMemcpyAsync(&s->ctrl->sphere, b, sizeof(lbm_sphere), hipMemcpyHostToDevice,
+                            s->stream));
+  // The copy reads *b asynchronously; wait so the caller may modify it.
+  return sync(s);
+}
+
+extern "C" int lbm_retag_cells(lbm_sim *s, const lbm_sphere *solid) {
+  CHECK_SIM(s);
+  int rc;
+  if (solid && (rc = upload_sphere(s, solid))) return rc;
+  rc = retag_full(s, solid ? 1 : 0);
+  if (rc) return rc;
+  s->need_full_retag = true;  // coupled path re-derives its band from scratch
+  return LBM_OK;
+}
+
+extern "C" int lbm_retag_cells_mask(lbm_sim *s, const uint8_t *mask, const float *us) {
+  CHECK_SIM(s);
+  if (!mask || !us) return fail(s, LBM_ERR_ARG, "mask/us is NULL");
+  const int64_t n = s->g.n;
+  if (!s->mask_dev) {
+    HIP_TRY(s, hipMalloc(&s->mask_dev, n));
+    HIP_TRY(s, hipMalloc(&s->mus_dev, n * 3 * sizeof(float)));
+  }
+  HIP_TRY(s, hipMemcpyAsync(s->mask_dev, mask, n, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(s, hipMemcpyAsync(s->mus_dev, us, n * 3 * sizeof(float), hipMemcpyHostToDevice, s->stream));
+  int rc = retag_full(s, 2);
+  if (rc) return rc;
+  s->need_full_retag = true;
+  return sync(s);
+}
+
+static int fresh_pass(lbm_sim *s, float *phi_fb, float *phi_w1, float *phi_w2, float *f, float *h) {
+  const lbm_params &p = s->params;
+  k_fresh<<<1, 1024, 0, s->stream>>>(s->g, p.rhoG, p.rhoL, s->fresh, s->ctrl, s->list_cap,
+                                     s->level_scratch, s->tag, phi_fb, phi_w1, phi_w2, s->rho,
+                                     s->ux, s->uy, s->uz, f, h);
+  return check_launch(s, "k_fresh");
+}
+
+extern "C" int lbm_handle_fresh_cells(lbm_sim *s) {
+  CHECK_SIM(s);
+  return fresh_pass(s, PHIPREV(s), PHI(s), PHIPREV(s), F(s), H(s));
+}
+
+extern "C" int lbm_compute_phi_gradients(lbm_sim *s) {
+  CHECK_SIM(s);
+  k_gradients<<<cell_grid(s), kCellBlock, 0, s->stream>>>(s->g, PHI(s), s->gx, s->gy, s->gz,
+                                                          s->lap);
+  s->grads_stale = false;
+  return check_launch(s, "k_gradients");
+}
+
+extern "C" int lbm_collide_phase(lbm_sim *s) {
+  CHECK_SIM(s);
+  const Derived d = derive(s->params);
+  k_collide_phase<<<cell_grid(s), kCellBlock, 0, s->stream>>>(s->g, d, H(s), H2(s), PHI(s), s->ux,
+                                                              s->uy, s->uz, s->tag, s->gx, s->gy,
+                                                              s->gz);
+  return check_launch(s, "k_collide_phase");
+}
+
+extern "C" int lbm_stream_phase(lbm_sim *s) {
+  CHECK_SIM(s);
+  k_stream_phase<<<cell_grid(s), kCellBlock, 0, s->stream>>>(s->g, H2(s), H(s), s->tag);
+  return check_launch(s, "k_stream_phase");
+}
+
+extern "C" int lbm_collide_hydro(lbm_sim *s) {
+  CHECK_SIM(s);
+  const Derived d = derive(s->params);
+  k_collide_hydro<<<cell_grid(s), kCellBlock, 0, s->stream>>>(
+      s->g, d, F(s), F2(s), s->rho, s->ux, s->uy, s->uz, PHI(s), s->tag, s->gx, s->gy, s->gz,
+      s->lap, s->bfx, s->bfy, s->bfz, s->bodyF_armed ? 1 : 0);
+  return check_launch(s, "k_collide_hydro");
+}
+
+// Runs streamHydroAndMacro; with emit, collects the link list (sorted by key) on
+// the host.  Clears the external body force afterwards (src/sim.js:581).
+static int stream_macro_pass(lbm_sim *s, bool emit, std::vector<int64_t> *keys,
+                             std::vector<double> *ms) {
+  const Derived d = derive(s->params);
+  int rc;
+  if (emit) {
+    // The tags may have been written by the host since the last retag, so count
+    // the solid cells now; they bound the number of links.
+    HIP_TRY(s, hipMemsetAsync(&s->ctrl->n_solid, 0, sizeof(int), s->stream));
+    k_count_solid<<<1024, 256, 0, s->stream>>>(s->g.n, s->tag, s->ctrl);
+    if ((rc = check_launch(s, "k_count_solid"))) return rc;
+    rc = read_ctrl(s);
+    if (rc) return rc;
+    const int64_t bound = std::max<int64_t>(1, (int64_t)s->ctrl_host->n_solid * 26);
+    if ((rc = ensure_links(s, bound))) return rc;
+    HIP_TRY(s, hipMemsetAsync(&s->ctrl->n_links, 0, sizeof(int), s->stream));
+  }
+  k_stream_hydro_macro<<<cell_grid(s), kCellBlock, 0, s->stream>>>(
+      s->g, d, F2(s), F(s), H(s), s->rho, s->ux, s->uy, s->uz, PHI(s), s->tag, s->usx, s->usy,
+      s->usz, s->gx, s->gy, s->gz, s->lap, s->bfx, s->bfy, s->bfz, s->bodyF_armed ? 1 : 0,
+      emit ? 1 : 0, s->link_key, s->link_m, s->link_cap, s->ctrl);
+  if ((rc = check_launch(s, "k_stream_hydro_macro"))) return rc;
+  if (s->bodyF_armed) {
+    const int64_t fb = s->g.S * sizeof(float);
+    HIP_TRY(s, hipMemsetAsync(s->bfx, 0, fb, s->stream));
+    HIP_TRY(s, hipMemsetAsync(s->bfy, 0, fb, s->stream));
+    HIP_TRY(s, hipMemsetAsync(s->bfz, 0, fb, s->stream));
+    s->bodyF_armed = false;
+  }
+  if (!emit) return LBM_OK;
+  if ((rc = read_ctrl(s))) return rc;
+  const int cnt = s->ctrl_host->n_links;
+  if (cnt > s->link_cap || (s->ctrl_host->err & kErrListOverflow))
+    return fail(s, LBM_ERR_CAPACITY, "momentum-exchange link list overflowed");
+  keys->resize(cnt);
+  ms->resize(cnt);
+  if (cnt) {
+    HIP_TRY(s, hipMemcpyAsync(keys->data(), s->link_key, cnt * sizeof(int64_t),
+                              hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(s, hipMemcpyAsync(ms->data(), s->link_m, cnt * sizeof(double), hipMemcpyDeviceToHost,
+                              s->stream));
+    if ((rc = sync(s))) return rc;
+  }
+  // Links were appended in arbitrary order; the reference visits cells in index
+  // order and directions k = 0..26 within a cell, which is ascending key order.
+  std::vector<int> ord(cnt);
+  for (int j = 0; j < cnt; j++) ord[j] = j;
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return (*keys)[a] < (*keys)[b]; });
+  std::vector<int64_t> k2(cnt);
+  std::vector<double> m2(cnt);
+  for (int j = 0; j < cnt; j++) {
+    k2[j] = (*keys)[ord[j]];
+    m2[j] = (*ms)[ord[j]];
+  }
+  keys->swap(k2);
+  ms->swap(m2);
+  return LBM_OK;
+}
+
+static void link_to_record(const lbm_sim *s, int64_t key, double m, lbm_link *L) {
+  const int64_t i = key / Q27;
+  const int k = (int)(key % Q27);
+  const int x = (int)(i % s->nx), y = (int)((i / s->nx) % s->ny), z = (int)(i / ((int64_t)s->nx * s->ny));
+  const double ekx = EX27[k], eky = EY27[k], ekz = EZ27[k];
+  // src/sim.js:533-535
+  L->x = x - EX27[k];
+  L->y = y - EY27[k];
+  L->z = z - EZ27[k];
+  L->k = k;
+  L->px = -m * ekx;
+  L->py = -m * eky;
+  L->pz = -m * ekz;
+}
+
+extern "C" int lbm_stream_hydro_macro(lbm_sim *s, lbm_sphere *solid) {
+  CHECK_SIM(s);
+  std::vector<int64_t> keys;
+  std::vector<double> ms;
+  int rc = stream_macro_pass(s, solid != nullptr, &keys, &ms);
+  if (rc) return rc;
+  if (solid) {
+    for (size_t j = 0; j < keys.size(); j++) {
+      lbm_link L;
+      link_to_record(s, keys[j], ms[j], &L);
+      sphere_apply_impulse(*solid, L.x, L.y, L.z, L.px, L.py, L.pz);
+    }
+  }
+  return sync(s);
+}
+
+extern "C" int lbm_stream_hydro_macro_links(lbm_sim *s, int *count) {
+  CHECK_SIM(s);
+  std::vector<int64_t> keys;
+  std::vector<double> ms;
+  int rc = stream_macro_pass(s, true, &keys, &ms);
+  if (rc) return rc;
+  s->links.resize(keys.size());
+  for (size_t j = 0; j < keys.size(); j++) link_to_record(s, keys[j], ms[j], &s->links[j]);
+  if (count) *count = (int)keys.size();
+  return LBM_OK;
+}
+
+extern "C" int lbm_get_links(lbm_sim *s, lbm_link *out, int cap) {
+  CHECK_SIM(s);
+  if (!out && cap > 0) return fail(s, LBM_ERR_ARG, "out is NULL");
+  const int m = std::min<int>(cap, (int)s->links.size());
+  for (int j = 0; j < m; j++) out[j] = s->links[j];
+  return LBM_OK;
+}
+
+extern "C" int lbm_sim_step(lbm_sim *s, lbm_sphere *solid) {
+  CHECK_SIM(s);
+  int rc;
+  if ((rc = lbm_retag_cells(s, solid))) return rc;
+  if ((rc = lbm_handle_fresh_cells(s))) return rc;
+  if ((rc = lbm_compute_phi_gradients(s))) return rc;
+  if ((rc = lbm_collide_phase(s))) return rc;
+  if ((rc = lbm_stream_phase(s))) return rc;
+  if ((rc = lbm_collide_hydro(s))) return rc;
+  return lbm_stream_hydro_macro(s, solid);
+}
+
+extern "C" int lbm_absorb_dead_cells(lbm_sim *s, lbm_sphere *solid) {
+  CHECK_SIM(s);
+  if (!solid) return fail(s, LBM_ERR_ARG, "solid is NULL");
+  int rc = read_ctrl(s);
+  if (rc) return rc;
+  const int cnt = std::min(s->ctrl_host->n_dead, s->list_cap);
+  if (cnt == 0) return LBM_OK;
+  if ((rc = ensure_imp(s, cnt))) return rc;
+  const lbm_params &p = s->params;
+  k_dead_impulses<<<(cnt + 255) / 256, 256, 0, s->stream>>>(s->dead, s->ctrl, s->list_cap, p.rhoG,
+                                                            p.rhoL - p.rhoG, PHI(s), s->ux, s->uy,
+                                                            s->uz, s->imp);
+  if ((rc = check_launch(s, "k_dead_impulses"))) return rc;
+  std::vector<int32_t> idx(cnt);
+  std::vector<double> imp(3 * (size_t)cnt);
+  HIP_TRY(s, hipMemcpyAsync(idx.data(), s->dead, cnt * sizeof(int32_t), hipMemcpyDeviceToHost,
+                            s->stream));
+  HIP_TRY(s, hipMemcpyAsync(imp.data(), s->imp, cnt * 3 * sizeof(double), hipMemcpyDeviceToHost,
+                            s->stream));
+  if ((rc = sync(s))) return rc;
+  // src/solid.js:90-99: index order.
+  for (int c = 0; c < cnt; c++) {
+    const int64_t i = idx[c];
+    const int x = (int)(i % s->nx), y = (int)((i / s->nx) % s->ny), z = (int)(i / ((int64_t)s->nx * s->ny));
+    sphere_apply_impulse(*solid, x, y, z, imp[3 * c], imp[3 * c + 1], imp[3 * c + 2]);
+  }
+  return LBM_OK;
+}
+
+// ------------------------------------------------------------------ fields
+
+extern "C" int64_t lbm_field_count(const lbm_sim *s, int field) {
+  if (!s) return -1;
+  switch (field) {
+    case LBM_FIELD_F:
+    case LBM_FIELD_F2: return s->g.n * 27;
+    case LBM_FIELD_H:
+    case LBM_FIELD_H2: return s->g.n * 7;
+    default: return (field >= 0 && field < LBM_FIELD_COUNT) ? s->g.n : -1;
+  }
+}
+
+static void *field_ptr(lbm_sim *s, int field, int *q, size_t *elem) {
+  *q = 1;
+  *elem = sizeof(float);
+  switch (field) {
+    case LBM_FIELD_F: *q = 27; return F(s);
+    case LBM_FIELD_F2: *q = 27; return F2(s);
+    case LBM_FIELD_H: *q = 7; return H(s);
+    case LBM_FIELD_H2: *q = 7; return H2(s);
+    case LBM_FIELD_RHO: return s->rho;
+    case LBM_FIELD_UX: return s->ux;
+    case LBM_FIELD_UY: return s->uy;
+    case LBM_FIELD_UZ: return s->uz;
+    case LBM_FIELD_PHI: return PHI(s);
+    case LBM_FIELD_PHIPREV: return PHIPREV(s);
+    case LBM_FIELD_TAG: *elem = 1; return s->tag;
+    case LBM_FIELD_TAGPREV: *elem = 1; return s->tagPrev;
+    case LBM_FIELD_USX: return s->usx;
+    case LBM_FIELD_USY: return s->usy;
+    case LBM_FIELD_USZ: return s->usz;
+    case LBM_FIELD_GX: return s->gx;
+    case LBM_FIELD_GY: return s->gy;
+    case LBM_FIELD_GZ: return s->gz;
+    case LBM_FIELD_LAP: return s->lap;
+    case LBM_FIELD_BODYFX: return s->bfx;
+    case LBM_FIELD_BODYFY: return s->bfy;
+    case LBM_FIELD_BODYFZ: return s->bfz;
+    default: return nullptr;
+  }
+}
+
+extern "C" int lbm_download(lbm_sim *s, int field, void *host) {
+  CHECK_SIM(s);
+  if (!host) return fail(s, LBM_ERR_ARG, "host pointer is NULL");
+  if (field >= LBM_FIELD_GX && field <= LBM_FIELD_LAP && s->grads_stale) {
+    // After a fused step the gradients of the last step are those of phiPrev
+    // (phi after that step's fresh-cell update).
+    k_gradients<<<cell_grid(s), kCellBlock, 0, s->stream>>>(s->g, PHIPREV(s), s->gx, s->gy,
+                                                            s->gz, s->lap);
+    int rc = check_launch(s, "k_gradients");
+    if (rc) return rc;
+    s->grads_stale = false;
+  }
+  int q;
+  size_t elem;
+  void *dp = field_ptr(s, field, &q, &elem);
+  if (!dp) return fail(s, LBM_ERR_ARG, "bad field id");
+  const int64_t n = s->g.n;
+  if (q == 1) {
+    HIP_TRY(s, hipMemcpyAsync(host, dp, n * elem, hipMemcpyDeviceToHost, s->stream));
+  } else {
+    int rc = ensure_scratch(s, n * q * sizeof(float));
+    if (rc) return rc;
+    k_soa_to_aos<<<(n + 255) / 256, 256, 0, s->stream>>>(n, s->g.S, q, (const float *)dp,
+                                                         (float *)s->scratch);
+    if ((rc = check_launch(s, "k_soa_to_aos"))) return rc;
+    HIP_TRY(s, hipMemcpyAsync(host, s->scratch, n * q * sizeof(float), hipMemcpyDeviceToHost,
+                              s->stream));
+  }
+  return sync(s);
+}
+
+extern "C" int lbm_upload(lbm_sim *s, int field, const void *host) {
+  CHECK_SIM(s);
+  if (!host) return fail(s, LBM_ERR_ARG, "host pointer is NULL");
+  int q;
+  size_t elem;
+  void *dp = field_ptr(s, field, &q, &elem);
+  if (!dp) return fail(s, LBM_ERR_ARG, "bad field id");
+  const int64_t n = s->g.n;
+  int rc;
+  if (q == 1) {
+    HIP_TRY(s, hipMemcpyAsync(dp, host, n * elem, hipMemcpyHostToDevice, s->stream));
+  } else {
+    if ((rc = ensure_scratch(s, n * q * sizeof(float)))) return rc;
+    HIP_TRY(s, hipMemcpyAsync(s->scratch, host, n * q * sizeof(float), hipMemcpyHostToDevice,
+                              s->stream));
+    k_aos_to_soa<<<(n + 255) / 256, 256, 0, s->stream>>>(n, s->g.S, q, (const float *)s->scratch,
+                                                         (float *)dp);
+    if ((rc = check_launch(s, "k_aos_to_soa"))) return rc;
+  }
+  if (field == LBM_FIELD_BODYFX || field == LBM_FIELD_BODYFY || field == LBM_FIELD_BODYFZ)
+    s->bodyF_armed = true;
+  if (field == LBM_FIELD_TAG || field == LBM_FIELD_TAGPREV || field == LBM_FIELD_PHI ||
+      field == LBM_FIELD_PHIPREV)
+    s->need_full_retag = true;
+  if (field >= LBM_FIELD_GX && field <= LBM_FIELD_LAP) s->grads_stale = false;
+  return sync(s);
+}
+
+extern "C" int lbm_pack_volume(lbm_sim *s, uint8_t *out) {
+  CHECK_SIM(s);
+  if (!out) return fail(s, LBM_ERR_ARG, "out is NULL");
+  const int64_t n = s->g.n;
+  int rc = ensure_scratch(s, 2 * n);
+  if (rc) return rc;
+  k_pack<<<(n + 255) / 256, 256, 0, s->stream>>>(n, PHI(s), s->tag, (uint8_t *)s->scratch);
+  if ((rc = check_launch(s, "k_pack"))) return rc;
+  HIP_TRY(s, hipMemcpyAsync(out, s->scratch, 2 * n, hipMemcpyDeviceToHost, s->stream));
+  return sync(s);
+}
+
+static int copy_list(lbm_sim *s, bool fresh, int32_t *out, int cap, int *count) {
+  int rc = read_ctrl(s);
+  if (rc) return rc;
+  const int cnt = std::min(fresh ? s->ctrl_host->n_fresh : s->ctrl_host->n_dead, s->list_cap);
+  if (count) *count = cnt;
+  const int m = std::min(cnt, cap);
+  if (m > 0 && out) {
+    HIP_TRY(s, hipMemcpyAsync(out, fresh ? s->fresh : s->dead, m * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, s->stream));
+    return sync(s);
+  }
+  return LBM_OK;
+}
+
+extern "C" int lbm_fresh_cells(lbm_sim *s, int32_t *out, int cap, int *count) {
+  CHECK_SIM(s);
+  return copy_list(s, true, out, cap, count);
+}
+
+extern "C" int lbm_dead_cells(lbm_sim *s, int32_t *out, int cap, int *count) {
+  CHECK_SIM(s);
+  return copy_list(s, false, out, cap, count);
+}
+
+// ------------------------------------------------------------------ sphere (host)
+
+extern "C" void lbm_sphere_init(lbm_sphere *b, int nx, int ny, int nz, double cx, double cy,
+                                double cz, double r, double density) {
+  if (!b) return;
+  std::memset(b, 0, sizeof *b);
+  b->nx = nx;
+  b->ny = ny;
+  b->nz = nz;
+  // src/solid.js:11-14, 24: `opts.x ?? default`.
+  b->cx = std::isnan(cx) ? nx * 0.5 : cx;
+  b->cy = std::isnan(cy) ? ny * 0.3 : cy;
+  b->cz = std::isnan(cz) ? nz * 0.5 : cz;
+  b->r = std::isnan(r) ? std::min(std::min(nx, ny), nz) * 0.10 : r;
+  b->density = std::isnan(density) ? 1.4 : density;
+  sphere_recompute_mass(*b);
+}
+
+extern "C" void lbm_sphere_recompute_mass(lbm_sphere *b) {
+  if (b) sphere_recompute_mass(*b);
+}
+
+extern "C" int lbm_sphere_contains(const lbm_sphere *b, int x, int y, int z) {
+  return b ? (sphere_contains(*b, x, y, z) ? 1 : 0) : 0;
+}
+
+extern "C" void lbm_sphere_surface_velocity(const lbm_sphere *b, int x, int y, int z,
+                                            double out[3]) {
+  if (b && out) sphere_surface_velocity(*b, x, y, z, out);
+}
+
+extern "C" void lbm_sphere_apply_impulse_at_cell(lbm_sphere *b, int x, int y, int z, double px,
+                                                 double py, double pz) {
+  if (b) sphere_apply_impulse(*b, x, y, z, px, py, pz);
+}
+
+extern "C" void lbm_sphere_step(lbm_sphere *b, double dt, double gravity) {
+  if (b) sphere_step(*b, dt, gravity);
+}
+
+extern "C" int lbm_lattice_tables(int32_t ex27[27], int32_t ey27[27], int32_t ez27[27],
+                                  float w27_[27], int32_t opp27_[27], int32_t ex7[7],
+                                  int32_t ey7[7], int32_t ez7[7], float w7_[7], int32_t opp7_[7]) {
+  for (int k = 0; k < 27; k++) {
+    ex27[k] = EX27[k];
+    ey27[k] = EY27[k];
+    ez27[k] = EZ27[k];
+    w27_[k] = w27(k);
+    opp27_[k] = opp27(k);
+  }
+  for (int k = 0; k < 7; k++) {
+    ex7[k] = EX7[k];
+    ey7[k] = EY7[k];
+    ez7[k] = EZ7[k];
+    w7_[k] = w7(k);
+    opp7_[k] = opp7(k);
+  }
+  return LBM_OK;
+}
+
+// ------------------------------------------------------------------ timing
+
+extern "C" int lbm_set_timing(lbm_sim *s, int on) {
+  CHECK_SIM(s);
+  int rc = sync(s);
+  if (rc) return rc;
+  collect_timing(s);
+  s->timing = on != 0;
+  s->ktime.clear();
+  return LBM_OK;
+}
+
+extern "C" int lbm_kernel_time(lbm_sim *s, const char *name, double *total_ms, int64_t *launches) {
+  CHECK_SIM(s);
+  if (!name) return LBM_ERR_ARG;
+  int rc = sync(s);
+  if (rc) return rc;
+  collect_timing(s);
+  auto it = s->ktime.find(name);
+  if (total_ms) *total_ms = it == s->ktime.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == s->ktime.end() ? 0 : it->second.second;
+  return LBM_OK;
+}
+
+// ------------------------------------------------------------------ coupled step
+
+#include "coupled.inc"

@@ -1,0 +1,321 @@
+"""ctypes binding of liblbm (include/lbm.h).
+
+Host-side plumbing for Python callers (tests, the benchmark): it loads
+lib/liblbm.so and exposes the C entry points with the same method names as the
+reference's Sim / RigidSphere (src/sim.js, src/solid.js).  All computation
+happens in the HIP library; this module only moves arguments."""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "liblbm.so")
+
+SPHERE_FIELDS = ["nx", "ny", "nz", "cx", "cy", "cz", "r", "vx", "vy", "vz", "wx", "wy", "wz",
+                 "density", "volume", "mass", "I", "dragging", "dragTargetX", "dragTargetY",
+                 "dragTargetZ", "Fx", "Fy", "Fz", "Tx", "Ty", "Tz"]
+PARAMS = ["tau", "tauPhi", "sigma", "gravity", "rhoL", "rhoG", "W", "mobility"]
+
+FIELD_IDS = {name: i for i, name in enumerate(
+    ["f", "f2", "h", "h2", "rho", "ux", "uy", "uz", "phi", "phiPrev", "tag", "tagPrev",
+     "usx", "usy", "usz", "gx", "gy", "gz", "lap", "bodyFx", "bodyFy", "bodyFz"])}
+_Q = {"f": 27, "f2": 27, "h": 7, "h2": 7}
+_U8 = {"tag", "tagPrev"}
+
+
+class LbmError(RuntimeError):
+    pass
+
+
+class Sphere(C.Structure):
+    """struct lbm_sphere: the fields of RigidSphere (src/solid.js:9-37)."""
+    _fields_ = [(k, C.c_double) for k in SPHERE_FIELDS]
+
+    def as_list(self):
+        return [getattr(self, k) for k in SPHERE_FIELDS]
+
+    def load_list(self, vals):
+        for k, v in zip(SPHERE_FIELDS, vals):
+            setattr(self, k, float(v))
+
+    def copy(self):
+        s = Sphere()
+        C.memmove(C.byref(s), C.byref(self), C.sizeof(Sphere))
+        return s
+
+
+class Params(C.Structure):
+    _fields_ = [(k, C.c_double) for k in PARAMS]
+
+
+class Link(C.Structure):
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32), ("z", C.c_int32), ("k", C.c_int32),
+                ("px", C.c_double), ("py", C.c_double), ("pz", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads liblbm; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LbmError(f"{LIB_PATH} is missing: run build.py (or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    SP = C.POINTER(Sphere)
+    sig = {
+        "lbm_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(P)]),
+        "lbm_destroy": (None, [P]),
+        "lbm_last_error": (C.c_char_p, [P]),
+        "lbm_dims": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "lbm_set_params": (C.c_int, [P, C.POINTER(Params)]),
+        "lbm_get_params": (C.c_int, [P, C.POINTER(Params)]),
+        "lbm_init_flat": (C.c_int, [P, C.c_double]),
+        "lbm_init_waterline_map": (C.c_int, [P, C.POINTER(C.c_double)]),
+        "lbm_seed_hydrostatic": (C.c_int, [P]),
+        "lbm_retag_cells": (C.c_int, [P, SP]),
+        "lbm_retag_cells_mask": (C.c_int, [P, C.POINTER(C.c_uint8), C.POINTER(C.c_float)]),
+        "lbm_handle_fresh_cells": (C.c_int, [P]),
+        "lbm_compute_phi_gradients": (C.c_int, [P]),
+        "lbm_collide_phase": (C.c_int, [P]),
+        "lbm_stream_phase": (C.c_int, [P]),
+        "lbm_collide_hydro": (C.c_int, [P]),
+        "lbm_stream_hydro_macro": (C.c_int, [P, SP]),
+        "lbm_stream_hydro_macro_links": (C.c_int, [P, C.POINTER(C.c_int)]),
+        "lbm_get_links": (C.c_int, [P, C.POINTER(Link), C.c_int]),
+        "lbm_sim_step": (C.c_int, [P, SP]),
+        "lbm_absorb_dead_cells": (C.c_int, [P, SP]),
+        "lbm_step": (C.c_int, [P, SP, C.c_int]),
+        "lbm_field_count": (C.c_int64, [P, C.c_int]),
+        "lbm_download": (C.c_int, [P, C.c_int, P]),
+        "lbm_upload": (C.c_int, [P, C.c_int, P]),
+        "lbm_pack_volume": (C.c_int, [P, C.POINTER(C.c_uint8)]),
+        "lbm_fresh_cells": (C.c_int, [P, C.POINTER(C.c_int32), C.c_int, C.POINTER(C.c_int)]),
+        "lbm_dead_cells": (C.c_int, [P, C.POINTER(C.c_int32), C.c_int, C.POINTER(C.c_int)]),
+        "lbm_sphere_init": (None, [SP, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_double]),
+        "lbm_sphere_recompute_mass": (None, [SP]),
+        "lbm_sphere_contains": (C.c_int, [SP, C.c_int, C.c_int, C.c_int]),
+        "lbm_sphere_surface_velocity": (None, [SP, C.c_int, C.c_int, C.c_int,
+                                               C.POINTER(C.c_double)]),
+        "lbm_sphere_apply_impulse_at_cell": (None, [SP, C.c_int, C.c_int, C.c_int, C.c_double,
+                                                    C.c_double, C.c_double]),
+        "lbm_sphere_step": (None, [SP, C.c_double, C.c_double]),
+        "lbm_lattice_tables": (C.c_int, [C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int32), C.POINTER(C.c_float),
+                                         C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
+        "lbm_stream": (P, [P]),
+        "lbm_synchronize": (C.c_int, [P]),
+        "lbm_set_timing": (C.c_int, [P, C.c_int]),
+        "lbm_kernel_time": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_int64)]),
+        "lbm_version": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def make_sphere(nx, ny, nz, cx=float("nan"), cy=float("nan"), cz=float("nan"),
+                r=float("nan"), density=float("nan")):
+    """new RigidSphere(nx, ny, nz, {cx, cy, cz, r, density}); NaN = default."""
+    b = Sphere()
+    lib().lbm_sphere_init(C.byref(b), nx, ny, nz, cx, cy, cz, r, density)
+    return b
+
+
+def sphere_step(b, dt, gravity):
+    lib().lbm_sphere_step(C.byref(b), dt, gravity)
+
+
+def recompute_mass(b):
+    lib().lbm_sphere_recompute_mass(C.byref(b))
+
+
+class Sim:
+    """A device-resident simulation (struct lbm_sim)."""
+
+    def __init__(self, nx, ny, nz):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.lbm_create(nx, ny, nz, C.byref(h))
+        if rc != 0:
+            raise LbmError(f"lbm_create({nx},{ny},{nz}) = {rc}: "
+                           f"{L.lbm_last_error(None).decode()}")
+        self.h = h
+        self.L = L
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.n = nx * ny * nz
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lbm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            raise LbmError(f"{what} = {rc}: {self.L.lbm_last_error(self.h).decode()}")
+
+    # -- parameters
+    def set_params(self, **kw):
+        p = Params()
+        self._ck(self.L.lbm_get_params(self.h, C.byref(p)), "lbm_get_params")
+        for k, v in kw.items():
+            if k not in PARAMS:
+                raise KeyError(k)
+            setattr(p, k, float(v))
+        self._ck(self.L.lbm_set_params(self.h, C.byref(p)), "lbm_set_params")
+
+    def params(self):
+        p = Params()
+        self._ck(self.L.lbm_get_params(self.h, C.byref(p)), "lbm_get_params")
+        return {k: getattr(p, k) for k in PARAMS}
+
+    # -- init
+    def init_flat(self, y):
+        self._ck(self.L.lbm_init_flat(self.h, float(y)), "lbm_init_flat")
+
+    def init_waterline_map(self, yw):
+        yw = np.ascontiguousarray(yw, dtype=np.float64)
+        assert yw.size == self.nx * self.nz
+        self._ck(self.L.lbm_init_waterline_map(self.h, yw.ctypes.data_as(C.POINTER(C.c_double))),
+                 "lbm_init_waterline_map")
+
+    def seed_hydrostatic(self):
+        self._ck(self.L.lbm_seed_hydrostatic(self.h), "lbm_seed_hydrostatic")
+
+    # -- passes (reference array semantics)
+    @staticmethod
+    def _sp(b):
+        return C.byref(b) if b is not None else None
+
+    def retag(self, b):
+        self._ck(self.L.lbm_retag_cells(self.h, self._sp(b)), "lbm_retag_cells")
+
+    def retag_mask(self, mask, us):
+        mask = np.ascontiguousarray(mask, np.uint8)
+        us = np.ascontiguousarray(us, np.float32)
+        self._ck(self.L.lbm_retag_cells_mask(self.h, mask.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                             us.ctypes.data_as(C.POINTER(C.c_float))),
+                 "lbm_retag_cells_mask")
+
+    def fresh(self):
+        self._ck(self.L.lbm_handle_fresh_cells(self.h), "lbm_handle_fresh_cells")
+
+    def gradients(self):
+        self._ck(self.L.lbm_compute_phi_gradients(self.h), "lbm_compute_phi_gradients")
+
+    def collide_phase(self):
+        self._ck(self.L.lbm_collide_phase(self.h), "lbm_collide_phase")
+
+    def stream_phase(self):
+        self._ck(self.L.lbm_stream_phase(self.h), "lbm_stream_phase")
+
+    def collide_hydro(self):
+        self._ck(self.L.lbm_collide_hydro(self.h), "lbm_collide_hydro")
+
+    def stream_hydro_macro(self, b):
+        self._ck(self.L.lbm_stream_hydro_macro(self.h, self._sp(b)), "lbm_stream_hydro_macro")
+
+    def stream_hydro_macro_links(self):
+        """streamHydroAndMacro for a generic solid: returns the ordered
+        applyImpulseAtCell calls as (x, y, z, k, px, py, pz) tuples."""
+        cnt = C.c_int(0)
+        self._ck(self.L.lbm_stream_hydro_macro_links(self.h, C.byref(cnt)),
+                 "lbm_stream_hydro_macro_links")
+        buf = (Link * max(cnt.value, 1))()
+        self._ck(self.L.lbm_get_links(self.h, buf, cnt.value), "lbm_get_links")
+        return [(l.x, l.y, l.z, l.k, l.px, l.py, l.pz) for l in buf[:cnt.value]]
+
+    def step(self, b):
+        """Sim.step(solid): the seven passes."""
+        self._ck(self.L.lbm_sim_step(self.h, self._sp(b)), "lbm_sim_step")
+
+    def absorb_dead(self, b):
+        self._ck(self.L.lbm_absorb_dead_cells(self.h, C.byref(b)), "lbm_absorb_dead_cells")
+
+    def coupled_passes(self, b, nsteps):
+        """nsteps of sim.step; absorbDeadCells; sphere step, pass by pass."""
+        for _ in range(nsteps):
+            self.step(b)
+            if b is not None:
+                self.absorb_dead(b)
+                sphere_step(b, 1.0, self.params()["gravity"])
+
+    def coupled(self, b, nsteps):
+        """nsteps fused coupled steps (lbm_step)."""
+        self._ck(self.L.lbm_step(self.h, self._sp(b), int(nsteps)), "lbm_step")
+
+    # -- fields
+    def get(self, name):
+        q = _Q.get(name, 1)
+        dt = np.uint8 if name in _U8 else np.float32
+        out = np.empty(self.n * q, dtype=dt)
+        self._ck(self.L.lbm_download(self.h, FIELD_IDS[name], out.ctypes.data_as(C.c_void_p)),
+                 f"lbm_download({name})")
+        return out
+
+    def set(self, name, arr):
+        q = _Q.get(name, 1)
+        dt = np.uint8 if name in _U8 else np.float32
+        a = np.ascontiguousarray(arr, dtype=dt)
+        assert a.size == self.n * q, (name, a.size)
+        self._ck(self.L.lbm_upload(self.h, FIELD_IDS[name], a.ctypes.data_as(C.c_void_p)),
+                 f"lbm_upload({name})")
+
+    def __getattr__(self, name):
+        if name in FIELD_IDS:
+            return self.get(name)
+        raise AttributeError(name)
+
+    def pack(self):
+        out = np.empty(2 * self.n, np.uint8)
+        self._ck(self.L.lbm_pack_volume(self.h, out.ctypes.data_as(C.POINTER(C.c_uint8))),
+                 "lbm_pack_volume")
+        return out
+
+    def _list(self, fn, what):
+        cnt = C.c_int(0)
+        self._ck(fn(self.h, None, 0, C.byref(cnt)), what)
+        out = np.empty(max(cnt.value, 1), np.int32)
+        self._ck(fn(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)), cnt.value, C.byref(cnt)),
+                 what)
+        return out[:cnt.value]
+
+    def fresh_cells(self):
+        return self._list(self.L.lbm_fresh_cells, "lbm_fresh_cells")
+
+    def dead_cells(self):
+        return self._list(self.L.lbm_dead_cells, "lbm_dead_cells")
+
+    # -- timing
+    def stream_handle(self):
+        return self.L.lbm_stream(self.h)
+
+    def synchronize(self):
+        self._ck(self.L.lbm_synchronize(self.h), "lbm_synchronize")
+
+    def set_timing(self, on):
+        self._ck(self.L.lbm_set_timing(self.h, 1 if on else 0), "lbm_set_timing")
+
+    def kernel_time(self, name):
+        ms = C.c_double(0)
+        cnt = C.c_int64(0)
+        self._ck(self.L.lbm_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(cnt)),
+                 "lbm_kernel_time")
+        return ms.value, cnt.value

@@ -1,0 +1,251 @@
+/*
+ * liblbm -- C ABI of the MI355X (gfx950) coupled lattice-Boltzmann step.
+ *
+ * One coupled step is the reference simulator's driver sequence
+ * (src/worker.js:96-100, test/bench.mjs:22-26):
+ *     sim.step(solid); solid.absorbDeadCells(sim); solid.step(1.0, sim.gravity)
+ * where Sim.step (src/sim.js:202-210) runs retag -> fresh cells -> phi
+ * gradients -> phase collide -> phase stream -> hydro collide -> hydro stream
+ * with macroscopic update.
+ *
+ * Every declaration below names the reference interface it replaces.  Plain C
+ * types only; all state lives in device memory owned by an lbm_sim handle.
+ * Functions return LBM_OK (0) or a negative LBM_ERR_* code, with a message
+ * available from lbm_last_error().  Calls on one handle must not overlap; each
+ * handle owns its own HIP stream.  Calls are synchronous unless stated.
+ *
+ * Field layout at the boundary is the reference's: cell i = (z*ny + y)*nx + x
+ * (x fastest), f as f[i*27 + k] and h as h[i*7 + k] in the direction order of
+ * src/lattices.js:34-49 and :79-84, float32 values, tags as uint8 (0 fluid,
+ * 1 solid, 2 wall).  The device keeps populations as structure-of-arrays.
+ */
+#ifndef LBM_H
+#define LBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_OK 0
+#define LBM_ERR_ARG (-1)      /* bad argument (null handle, bad size, bad field id) */
+#define LBM_ERR_OOM (-2)      /* device or host allocation failed */
+#define LBM_ERR_HIP (-3)      /* a HIP runtime call failed */
+#define LBM_ERR_STATE (-4)    /* call not valid in the current state */
+#define LBM_ERR_CAPACITY (-5) /* a bounded list overflowed */
+#define LBM_ERR_NODEVICE (-6) /* no usable GPU */
+
+/* Cell tags, src/lattices.js:102-104. */
+#define LBM_TAG_FLUID 0
+#define LBM_TAG_SOLID 1
+#define LBM_TAG_WALL 2
+
+typedef struct lbm_sim lbm_sim;
+
+/* Physics parameters: the public scalar fields of class Sim (src/sim.js:88-95).
+ * They are read at every pass, so they may change between steps
+ * (src/worker.js:139-145).  mobility is carried but unused, as in the
+ * reference: the phase mobility is cs7^2 (tauPhi - 1/2) (src/sim.js:329). */
+typedef struct lbm_params {
+  double tau, tauPhi, sigma, gravity, rhoL, rhoG, W, mobility;
+} lbm_params;
+
+/* The fields of class RigidSphere (src/solid.js:9-37), all FP64, in this order.
+ * dragging is 0 or 1.  Fx..Tz are the momentum-exchange accumulators. */
+typedef struct lbm_sphere {
+  double nx, ny, nz;
+  double cx, cy, cz, r;
+  double vx, vy, vz;
+  double wx, wy, wz;
+  double density, volume, mass, I;
+  double dragging, dragTargetX, dragTargetY, dragTargetZ;
+  double Fx, Fy, Fz, Tx, Ty, Tz;
+} lbm_sphere;
+
+/* Field identifiers for lbm_download / lbm_upload: the typed arrays of class
+ * Sim (src/sim.js:35-64). */
+enum lbm_field {
+  LBM_FIELD_F = 0,  /* n*27 float */
+  LBM_FIELD_F2,     /* n*27 float */
+  LBM_FIELD_H,      /* n*7 float */
+  LBM_FIELD_H2,     /* n*7 float */
+  LBM_FIELD_RHO,    /* n float: the pressure proxy sum(f) */
+  LBM_FIELD_UX,
+  LBM_FIELD_UY,
+  LBM_FIELD_UZ,
+  LBM_FIELD_PHI,
+  LBM_FIELD_PHIPREV,
+  LBM_FIELD_TAG,    /* n uint8 */
+  LBM_FIELD_TAGPREV,/* n uint8 */
+  LBM_FIELD_USX,
+  LBM_FIELD_USY,
+  LBM_FIELD_USZ,
+  LBM_FIELD_GX,
+  LBM_FIELD_GY,
+  LBM_FIELD_GZ,
+  LBM_FIELD_LAP,
+  LBM_FIELD_BODYFX,
+  LBM_FIELD_BODYFY,
+  LBM_FIELD_BODYFZ,
+  LBM_FIELD_COUNT
+};
+
+/* ------------------------------------------------------------------ lifetime */
+
+/* new Sim(nx, ny, nz) (src/sim.js:30-96): allocates the device state and sets
+ * the reference's default parameters.  The state is zero until an init call. */
+int lbm_create(int nx, int ny, int nz, lbm_sim **out);
+void lbm_destroy(lbm_sim *sim);
+/* Message for the last failed call on this handle (or the last failed
+ * lbm_create when sim is NULL). */
+const char *lbm_last_error(const lbm_sim *sim);
+/* Grid size; n = nx*ny*nz. */
+int lbm_dims(const lbm_sim *sim, int *nx, int *ny, int *nz);
+
+/* Assignments to sim.tau, sim.tauPhi, ... (src/sim.js:88-95). */
+int lbm_set_params(lbm_sim *sim, const lbm_params *p);
+int lbm_get_params(const lbm_sim *sim, lbm_params *p);
+
+/* ------------------------------------------------------------------ init */
+
+/* Sim.initFlat(waterlineY) (src/sim.js:112-134), including its call of
+ * seedHydrostatic. */
+int lbm_init_flat(lbm_sim *sim, double waterline_y);
+/* initFlat with one waterline per (x,z) column, yw[z*nx + x]; a constant map
+ * gives exactly lbm_init_flat.  No reference counterpart: it seeds perturbed
+ * interfaces. */
+int lbm_init_waterline_map(lbm_sim *sim, const double *yw);
+/* Sim.seedHydrostatic() (src/sim.js:144-173). */
+int lbm_seed_hydrostatic(lbm_sim *sim);
+
+/* ------------------------------------------------------------------ passes */
+/* The seven passes of Sim.step, one entry each, with the reference's array
+ * semantics (f -> f2 -> f, h -> h2 -> h, stored gradients).  `solid` may be
+ * NULL, as sim.step(null) allows (src/sim.js:233, 527). */
+
+/* Sim.retagCells(solid) (src/sim.js:214-244) for a RigidSphere. */
+int lbm_retag_cells(lbm_sim *sim, const lbm_sphere *solid);
+/* Sim.retagCells(solid) for any other solid: the caller evaluated
+ * solid.contains (mask[i] != 0 on interior cells) and solid.surfaceVelocity
+ * (us[3*i+c], stored as float like the reference) on the host. */
+int lbm_retag_cells_mask(lbm_sim *sim, const uint8_t *mask, const float *us);
+/* Sim.handleFreshCells(solid) (src/sim.js:257-310); uses the fresh cells
+ * found by the last retag. */
+int lbm_handle_fresh_cells(lbm_sim *sim);
+/* Sim.computePhiGradients() (src/sim.js:176-199). */
+int lbm_compute_phi_gradients(lbm_sim *sim);
+/* Sim.collidePhase() (src/sim.js:324-364). */
+int lbm_collide_phase(lbm_sim *sim);
+/* Sim.streamPhase() (src/sim.js:376-397). */
+int lbm_stream_phase(lbm_sim *sim);
+/* Sim.collideHydro() (src/sim.js:406-464). */
+int lbm_collide_hydro(lbm_sim *sim);
+/* Sim.streamHydroAndMacro(solid) (src/sim.js:484-582).  With a sphere, every
+ * fluid->solid link's impulse is added to solid->F/T through
+ * lbm_sphere_apply_impulse_at_cell in the reference's cell-then-direction
+ * order, so the accumulators match the reference bit for bit. */
+int lbm_stream_hydro_macro(lbm_sim *sim, lbm_sphere *solid);
+
+/* One momentum-exchange record of lbm_stream_hydro_macro_links. */
+typedef struct lbm_link {
+  int32_t x, y, z;    /* solid (source) cell, the applyImpulseAtCell position */
+  int32_t k;          /* lattice direction of the fluid population */
+  double px, py, pz;  /* impulse handed to the body */
+} lbm_link;
+/* streamHydroAndMacro for a solid that is not a RigidSphere: instead of
+ * accumulating into a sphere, the handle keeps every link's applyImpulseAtCell
+ * call in reference order (src/sim.js:527-536) so the host can replay them on
+ * its own object.  *count receives the number of links; lbm_get_links copies
+ * them out. */
+int lbm_stream_hydro_macro_links(lbm_sim *sim, int *count);
+/* Copies min(count, cap) links kept by the last lbm_stream_hydro_macro_links. */
+int lbm_get_links(lbm_sim *sim, lbm_link *out, int cap);
+
+/* Sim.step(solid) (src/sim.js:202-210): the seven passes above. */
+int lbm_sim_step(lbm_sim *sim, lbm_sphere *solid);
+
+/* RigidSphere.absorbDeadCells(sim) (src/solid.js:87-101): adds each dead cell's
+ * impulse (cells that became solid at the last retag), in index order. */
+int lbm_absorb_dead_cells(lbm_sim *sim, lbm_sphere *solid);
+
+/* ------------------------------------------------------------------ coupled */
+
+/* nsteps coupled steps (src/worker.js:96-100): per step sim.step(solid);
+ * solid.absorbDeadCells(sim); solid.step(1.0, sim.gravity).  This is the fast
+ * path: the sphere is integrated on the device, passes are fused, and the host
+ * synchronises once per call.  Field results equal lbm_sim_step's; the force
+ * sums use a fixed parallel order, so the sphere state can differ from the
+ * per-pass path in the last bits.  solid may be NULL (no body). */
+int lbm_step(lbm_sim *sim, lbm_sphere *solid, int nsteps);
+
+/* ------------------------------------------------------------------ fields */
+
+/* Number of elements of a field (n, n*27 or n*7). */
+int64_t lbm_field_count(const lbm_sim *sim, int field);
+/* Reads of the Sim typed arrays (sim.phi[i] etc.), in the reference layout. */
+int lbm_download(lbm_sim *sim, int field, void *host);
+/* Writes to the Sim typed arrays (sim.phi[i] = ..., src/worker.js:61-67).  A
+ * write to bodyF arms the external force for the next step; the step clears it
+ * (src/sim.js:581). */
+int lbm_upload(lbm_sim *sim, int field, const void *host);
+/* Sim.packVolume(out) (src/sim.js:586-600): out has 2n bytes, R = phi*255
+ * truncated, G = 255 on solid cells. */
+int lbm_pack_volume(lbm_sim *sim, uint8_t *out);
+/* Cells that turned fluid (fresh) or solid (dead) at the last retag, ascending.
+ * Writes min(count, cap) entries; *count gets the full count. */
+int lbm_fresh_cells(lbm_sim *sim, int32_t *out, int cap, int *count);
+int lbm_dead_cells(lbm_sim *sim, int32_t *out, int cap, int *count);
+
+/* ------------------------------------------------------------------ sphere */
+/* Host implementations of class RigidSphere (src/solid.js), compiled from the
+ * same source as the device integrator. */
+
+/* new RigidSphere(nx, ny, nz, opts) (src/solid.js:9-37).  Pass NaN for an
+ * option to take the reference default (cx = nx/2, cy = 0.3 ny, cz = nz/2,
+ * r = 0.1 min(nx,ny,nz), density 1.4). */
+void lbm_sphere_init(lbm_sphere *s, int nx, int ny, int nz, double cx, double cy, double cz,
+                     double r, double density);
+/* RigidSphere.recomputeMass (src/solid.js:39-44); setRadius/setDensity assign
+ * the field and call it. */
+void lbm_sphere_recompute_mass(lbm_sphere *s);
+/* RigidSphere.contains (src/solid.js:50-55). */
+int lbm_sphere_contains(const lbm_sphere *s, int x, int y, int z);
+/* RigidSphere.surfaceVelocity (src/solid.js:57-66). */
+void lbm_sphere_surface_velocity(const lbm_sphere *s, int x, int y, int z, double out[3]);
+/* RigidSphere.applyImpulseAtCell (src/solid.js:70-80). */
+void lbm_sphere_apply_impulse_at_cell(lbm_sphere *s, int x, int y, int z, double px,
+                                      double py, double pz);
+/* RigidSphere.step(dt, gravity) (src/solid.js:104-210). */
+void lbm_sphere_step(lbm_sphere *s, double dt, double gravity);
+
+/* ------------------------------------------------------------------ tables */
+
+/* The lattice tables (src/lattices.js:16-99) as the library uses them. */
+int lbm_lattice_tables(int32_t ex27[27], int32_t ey27[27], int32_t ez27[27], float w27[27],
+                       int32_t opp27[27], int32_t ex7[7], int32_t ey7[7], int32_t ez7[7],
+                       float w7[7], int32_t opp7[7]);
+
+/* ------------------------------------------------------------------ timing */
+
+/* The handle's HIP stream (a hipStream_t), for callers that time work with
+ * HIP events on it. */
+void *lbm_stream(lbm_sim *sim);
+/* Blocks until the handle's queued work is done. */
+int lbm_synchronize(lbm_sim *sim);
+/* When on, lbm_step records HIP events around each launch of its kernels and
+ * accumulates per-kernel times, read with lbm_kernel_time. */
+int lbm_set_timing(lbm_sim *sim, int on);
+/* Total milliseconds and launch count of a kernel since timing was turned on.
+ * Names: "stream_collide", "phase_stream", "band", "body", "collide_all",
+ * "stream_macro". */
+int lbm_kernel_time(lbm_sim *sim, const char *name, double *total_ms, int64_t *launches);
+
+/* Library version string. */
+const char *lbm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_H */
