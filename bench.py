@@ -82,11 +82,17 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-only", action="store_true",
+                    help="only run the plain copy kernel (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the coupled step on a single MI355X "
               "(multi-GPU sharding is a design note only).", file=sys.stderr)
         return 2
+    copy_gbps = lbm.copy_bandwidth(1 << 31, 10)
+    if args.copy_probe_only:
+        print(json.dumps({"copy_probe_gbps": round(copy_gbps, 1), "bytes": 1 << 31, "reps": 10}))
+        return 0
     n = args.size
     sim, b, params, yw, sph = build_scene(n, args.seed)
     print(f"[bench] scene {n}^3 built; warmup {args.warmup}", file=sys.stderr, flush=True)
@@ -133,6 +139,7 @@ def main():
                      "bytes_per_lu": BYTES_PER_LU["stream_collide"],
                      "avg_ms": round(avg_s * 1e3, 4)},
         "whole_step_hbm_frac": round(297 * mlups * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
+        "copy_probe_gbps": round(copy_gbps, 1),
         "kernel_ms": {k: {"total": round(v[0], 3), "launches": v[1]} for k, v in kt.items()},
         "finite": finite,
         "sphere": {"cx": b.cx, "cy": b.cy, "cz": b.cz, "vx": b.vx, "vy": b.vy, "vz": b.vz},

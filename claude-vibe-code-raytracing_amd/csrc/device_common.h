@@ -41,6 +41,39 @@ constexpr int kErrListOverflow = 1;
 constexpr int kErrBandEscape = 2;
 constexpr int kErrFreshCapacity = 4;
 
+// Tiles of 64 x 4 x 1 cells (one wave per x-row segment) launched on a 1-D grid
+// in XCD-aware order.  The hardware hands consecutive workgroups to the eight
+// XCDs in turn, and each XCD has its own L2; mapping physical workgroup p to
+// tile (p % 8) * (T8 / 8) + p / 8 gives every XCD a contiguous run of tiles (a
+// slab of z-planes), so tiles that share halo rows and cache lines run on the
+// same L2.
+struct TileMap {
+  int tx, ty, tz;  // tiles per axis
+  int T;           // tx * ty * tz
+  int T8;          // T rounded up to a multiple of 8 (the launched grid)
+};
+
+inline TileMap make_tile_map(int nx, int ny, int nz) {
+  TileMap m;
+  m.tx = (nx + 63) / 64;
+  m.ty = (ny + 3) / 4;
+  m.tz = nz;
+  m.T = m.tx * m.ty * m.tz;
+  m.T8 = (m.T + 7) / 8 * 8;
+  return m;
+}
+
+// Tile of this workgroup; false for the padding workgroups past T.
+__device__ __forceinline__ bool map_tile(const TileMap &m, int &tile, int &bx, int &by, int &bz) {
+  const int p = blockIdx.x;
+  tile = (p & 7) * (m.T8 >> 3) + (p >> 3);
+  if (tile >= m.T) return false;
+  bx = tile % m.tx;
+  by = (tile / m.tx) % m.ty;
+  bz = tile / (m.tx * m.ty);
+  return true;
+}
+
 __device__ __forceinline__ int64_t cell_index(const Geom &g, int x, int y, int z) {
   return ((int64_t)z * g.ny + y) * g.nx + x;
 }

@@ -61,13 +61,16 @@ __device__ __forceinline__ bool in_box(const int box[6], int x, int y, int z) {
 
 // phi(t+1) for every cell: fluid cells sum their pulled h (src/sim.js:376-397,
 // 549-552); other cells keep phi (the reference never updates them).
-__global__ __launch_bounds__(256) void k_phase_stream(Geom g, const float *__restrict__ h2,
+__global__ __launch_bounds__(256) void k_phase_stream(Geom g, TileMap tm,
+                                                      const float *__restrict__ h2,
                                                       const uint8_t *__restrict__ tag,
                                                       const float *__restrict__ phi,
                                                       float *__restrict__ phi_next) {
-  const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * 4 + threadIdx.y;
-  const int z = blockIdx.z;
+  int tile, bx, by, bz;
+  if (!map_tile(tm, tile, bx, by, bz)) return;
+  const int x = bx * 64 + threadIdx.x;
+  const int y = by * 4 + threadIdx.y;
+  const int z = bz;
   if (x >= g.nx || y >= g.ny) return;
   const int64_t i = cell_index(g, x, y, z);
   if (tag[i] != TAG_FLUID || !interior(g, x, y, z)) {
@@ -292,21 +295,24 @@ __device__ __forceinline__ void pull27(const Geom &g, int64_t i, const float *__
   }
 }
 
-// Deterministic block sum of 6 doubles (fixed butterfly within each wave, waves
-// in order), written to partials[block] and the block id appended to the list.
-__device__ __forceinline__ void block_momentum(double mom[6], bool linked, double *partials,
-                                               int32_t *plist, int *pcount, int64_t block) {
+// Sum of a double over the 64 lanes of a wave by a fixed butterfly; every lane
+// receives the same total.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Link momentum of one 64x4 tile (6 doubles), summed in a fixed order (butterfly
+// within each wave, waves in order) and stored at partials[tile].  Every tile
+// that intersects the band box stores its sum, zero or not, so the body update
+// can read exactly the band's tiles without a list.
+__device__ __forceinline__ void tile_momentum(double mom[6], double *partials, int64_t tile) {
   __shared__ double red[4][6];
   const int lane = threadIdx.x;  // 0..63
   const int wave = threadIdx.y;  // 0..3
-  if (!__syncthreads_or(linked ? 1 : 0)) return;
 #pragma unroll
-  for (int c = 0; c < 6; c++) {
-    double v = mom[c];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    mom[c] = v;
-  }
+  for (int c = 0; c < 6; c++) mom[c] = wave_sum(mom[c]);
   if (lane == 0) {
 #pragma unroll
     for (int c = 0; c < 6; c++) red[wave][c] = mom[c];
@@ -315,9 +321,7 @@ __device__ __forceinline__ void block_momentum(double mom[6], bool linked, doubl
   if (lane == 0 && wave == 0) {
 #pragma unroll
     for (int c = 0; c < 6; c++)
-      partials[block * 6 + c] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
-    const int slot = atomicAdd(pcount, 1);
-    plist[slot] = (int32_t)block;
+      partials[tile * 6 + c] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
   }
 }
 
@@ -331,20 +335,27 @@ __device__ __forceinline__ void block_momentum(double mom[6], bool linked, doubl
 // without).
 template <bool COLLIDE>
 __global__ __launch_bounds__(256) void k_stream_collide(
-    Geom g, Derived d, const float *__restrict__ f2, const float *__restrict__ h2,
+    Geom g, TileMap tm, Derived d, const float *__restrict__ f2, const float *__restrict__ h2,
     float *__restrict__ fo, float *__restrict__ ho, const float *__restrict__ phi,
     float *__restrict__ phi_next, float *__restrict__ rho, float *__restrict__ ux,
     float *__restrict__ uy, float *__restrict__ uz, const uint8_t *__restrict__ tag,
     const float *__restrict__ usx, const float *__restrict__ usy, const float *__restrict__ usz,
     const float *__restrict__ bfx, const float *__restrict__ bfy, const float *__restrict__ bfz,
-    int use_bf, const Ctrl *__restrict__ ctrl, int has_solid, double *__restrict__ partials,
-    int32_t *__restrict__ plist, int *__restrict__ pcount) {
-  const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * 4 + threadIdx.y;
-  const int z = blockIdx.z;
+    int use_bf, const Ctrl *__restrict__ ctrl, int has_solid, double *__restrict__ partials) {
+  int tile, bx, by, bz;
+  if (!map_tile(tm, tile, bx, by, bz)) return;
+  const int x = bx * 64 + threadIdx.x;
+  const int y = by * 4 + threadIdx.y;
+  const int z = bz;
   double mom[6] = {0, 0, 0, 0, 0, 0};
   bool linked = false;
-  const bool exchange = has_solid != 0;
+  int box[6];
+#pragma unroll
+  for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
+  // Link momentum can only arise next to the sphere, which lies inside the band.
+  const bool tile_in_band = bx * 64 <= box[1] && bx * 64 + 63 >= box[0] && by * 4 <= box[3] &&
+                            by * 4 + 3 >= box[2] && bz >= box[4] && bz <= box[5];
+  const bool exchange = has_solid != 0 && tile_in_band;
   double cx = 0, cy = 0, cz = 0;
   if (exchange) {
     cx = ctrl->sphere.cx;
@@ -402,16 +413,19 @@ __global__ __launch_bounds__(256) void k_stream_collide(
       }
       const float rhof = (float)rhoN, phif = (float)p;
       const float uxf = (float)uxn, uyf = (float)uyn, uzf = (float)uzn;
-      rho[i] = rhof;
-      ux[i] = uxf;
-      uy[i] = uyf;
-      uz[i] = uzf;
+      const bool band = in_box(box, x, y, z);
+      // rho and u are read before the end of the call only inside the band (by the
+      // fresh-cell pass, the dead-cell impulses and the band collide); the last
+      // step of a call stores them everywhere.
+      if (!COLLIDE || band) {
+        rho[i] = rhof;
+        ux[i] = uxf;
+        uy[i] = uyf;
+        uz[i] = uzf;
+      }
       bool store_stream = true;
       if (COLLIDE) {
-        int box[6];
-#pragma unroll
-        for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
-        if (!in_box(box, x, y, z)) {
+        if (!band) {
           store_stream = false;
           // collide(t+1) with the new macroscopic values (rounded to float as the
           // reference stores them) and the gradients of phi(t+1).
@@ -440,10 +454,8 @@ __global__ __launch_bounds__(256) void k_stream_collide(
       }
     }
   }
-  if (exchange) {
-    const int64_t block = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-    block_momentum(mom, linked, partials, plist, pcount, block);
-  }
+  (void)linked;
+  if (exchange) tile_momentum(mom, partials, tile);
 }
 
 // Gradients + collide for the fluid cells of a box, reading post-stream
@@ -489,34 +501,19 @@ __global__ __launch_bounds__(256) void k_collide_region(
   for (int k = 0; k < Q27; k++) fo[k * g.S + i] = fc[k];
 }
 
-// Deterministic sum of one double per thread over a 1024-thread block: pairs are
-// combined in a fixed tree, so the result does not depend on scheduling.
-__device__ __forceinline__ double block_tree_sum(double v, double *red) {
-  const int tid = threadIdx.x;
-  red[tid] = v;
-  __syncthreads();
-  for (int w = 512; w > 0; w >>= 1) {
-    if (tid < w) red[tid] = red[tid] + red[tid + w];
-    __syncthreads();
-  }
-  const double r = red[0];
-  __syncthreads();
-  return r;
-}
-
 // Body update after stream(t), one 1024-thread workgroup:
-//   F/T += link momentum, summed over the blocks that touched the body in
-//          ascending block order (contiguous chunks per thread, fixed tree);
-//   F/T += dead-cell impulses rho_phase(phi) u at each cell that turned solid at
-//          this step's retag (src/solid.js:87-101), same chunked fixed order;
+//   F/T += link momentum of the tiles intersecting this step's band box, in
+//          ascending tile order (contiguous chunks per thread, then waves in
+//          order);
+//   F/T += dead-cell impulses rho_phase(phi) u at the cells that turned solid at
+//          this step's retag (src/solid.js:87-101), in the same chunked order;
 //   RigidSphere.step(1, g) (src/solid.js:104-210).
 // Then rolls the band boxes (the box of the old position serves the band
 // kernels of the next step, the box of the new position the next fused
 // launch) and, when check_band is set, verifies the centre moved at most
 // kBandDelta per axis.
-__global__ __launch_bounds__(1024) void k_body(Geom g, Ctrl *ctrl,
-                                               const double *__restrict__ partials, int32_t *plist,
-                                               int *pcount, int plist_cap,
+__global__ __launch_bounds__(1024) void k_body(Geom g, TileMap tm, Ctrl *ctrl,
+                                               const double *__restrict__ partials,
                                                const int32_t *__restrict__ dead, int dead_cap,
                                                const float *__restrict__ phi,
                                                const float *__restrict__ ux,
@@ -524,64 +521,31 @@ __global__ __launch_bounds__(1024) void k_body(Geom g, Ctrl *ctrl,
                                                const float *__restrict__ uz, double rhoG,
                                                double dRho, double gravity, int check_band,
                                                int reset_lists) {
-  __shared__ int32_t sh[kSortLds];
-  __shared__ double red[1024];
+  __shared__ double red[16][12];
   const int tid = threadIdx.x;
-  int count = *pcount;
-  if (count > plist_cap) count = plist_cap;
-  int P = 1;
-  while (P < count) P <<= 1;
-  int32_t *w = plist;
-  if (count > 1) {
-    if (P <= kSortLds) {
-      for (int t = tid; t < P; t += blockDim.x) sh[t] = t < count ? plist[t] : INT32_MAX;
-      w = sh;
-    } else {
-      for (int t = count + tid; t < P; t += blockDim.x) plist[t] = INT32_MAX;
-    }
-    __threadfence_block();
-    __syncthreads();
-    for (int k = 2; k <= P; k <<= 1) {
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int t = tid; t < P; t += blockDim.x) {
-          const int u = t ^ j;
-          if (u > t) {
-            const int32_t at = w[t], au = w[u];
-            const bool up = (t & k) == 0;
-            if ((at > au) == up) {
-              w[t] = au;
-              w[u] = at;
-            }
-          }
-        }
-        __threadfence_block();
-        __syncthreads();
-      }
-    }
-  } else if (count == 1) {
-    if (tid == 0) sh[0] = plist[0];
-    w = sh;
-    __syncthreads();
-  }
   const lbm_sphere b0 = ctrl->sphere;
-  // Link momentum: thread t sums sorted entries [t*C, (t+1)*C) in order.
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  {
-    const int C = (count + 1023) / 1024;
-    const int lo = tid * C, hi = min(count, lo + C);
-    for (int c = lo; c < hi; c++) {
-      const int64_t blk = w[c];
+  int box[6];
 #pragma unroll
-      for (int q = 0; q < 6; q++) acc[q] += partials[blk * 6 + q];
+  for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
+  const int bx0 = box[0] >> 6, nbx = (box[1] >> 6) - bx0 + 1;
+  const int by0 = box[2] >> 2, nby = (box[3] >> 2) - by0 + 1;
+  const int bz0 = box[4], nbz = box[5] - box[4] + 1;
+  const int ntiles = nbx * nby * nbz;
+  double acc[12];
+#pragma unroll
+  for (int q = 0; q < 12; q++) acc[q] = 0;
+  {
+    const int C = (ntiles + 1023) / 1024;
+    const int lo = tid * C, hi = min(ntiles, lo + C);
+    for (int c = lo; c < hi; c++) {
+      const int tx_ = c % nbx, ty_ = (c / nbx) % nby, tz_ = c / (nbx * nby);
+      const int64_t t = (int64_t)(bx0 + tx_) + (int64_t)tm.tx * ((by0 + ty_) + (int64_t)tm.ty * (bz0 + tz_));
+#pragma unroll
+      for (int q = 0; q < 6; q++) acc[q] += partials[t * 6 + q];
     }
   }
-  double link[6];
-#pragma unroll
-  for (int q = 0; q < 6; q++) link[q] = block_tree_sum(acc[q], red);
-  // Dead-cell impulses, applyImpulseAtCell at the cell centre.
   int nd = ctrl->n_dead;
   if (nd > dead_cap) nd = dead_cap;
-  double dacc[6] = {0, 0, 0, 0, 0, 0};
   {
     const int C = (nd + 1023) / 1024;
     const int lo = tid * C, hi = min(nd, lo + C);
@@ -595,33 +559,42 @@ __global__ __launch_bounds__(1024) void k_body(Geom g, Ctrl *ctrl,
       const double rx = x + 0.5 - b0.cx;
       const double ry = y + 0.5 - b0.cy;
       const double rz = z + 0.5 - b0.cz;
-      dacc[0] += px;
-      dacc[1] += py;
-      dacc[2] += pz;
-      dacc[3] += ry * pz - rz * py;
-      dacc[4] += rz * px - rx * pz;
-      dacc[5] += rx * py - ry * px;
+      acc[6] += px;
+      acc[7] += py;
+      acc[8] += pz;
+      acc[9] += ry * pz - rz * py;
+      acc[10] += rz * px - rx * pz;
+      acc[11] += rx * py - ry * px;
     }
   }
-  double dsum[6];
+  const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
-  for (int q = 0; q < 6; q++) dsum[q] = block_tree_sum(dacc[q], red);
+  for (int q = 0; q < 12; q++) {
+    const double v = wave_sum(acc[q]);
+    if (lane == 0) red[wave][q] = v;
+  }
+  __syncthreads();
   if (tid != 0) return;
+  double tot[12];
+  for (int q = 0; q < 12; q++) {
+    double v = 0;
+    for (int w = 0; w < 16; w++) v += red[w][q];
+    tot[q] = v;
+  }
   lbm_sphere b = b0;
-  b.Fx += link[0];
-  b.Fy += link[1];
-  b.Fz += link[2];
-  b.Tx += link[3];
-  b.Ty += link[4];
-  b.Tz += link[5];
-#pragma unroll
-  for (int q = 0; q < 6; q++) ctrl->F_links[q] = link[q];
-  b.Fx += dsum[0];
-  b.Fy += dsum[1];
-  b.Fz += dsum[2];
-  b.Tx += dsum[3];
-  b.Ty += dsum[4];
-  b.Tz += dsum[5];
+  b.Fx += tot[0];
+  b.Fy += tot[1];
+  b.Fz += tot[2];
+  b.Tx += tot[3];
+  b.Ty += tot[4];
+  b.Tz += tot[5];
+  for (int q = 0; q < 6; q++) ctrl->F_links[q] = tot[q];
+  b.Fx += tot[6];
+  b.Fy += tot[7];
+  b.Fz += tot[8];
+  b.Tx += tot[9];
+  b.Ty += tot[10];
+  b.Tz += tot[11];
   sphere_step(b, 1.0, gravity);
   if (check_band) {
     const double d0 = b.cx - b0.cx, d1 = b.cy - b0.cy, d2 = b.cz - b0.cz;
@@ -630,7 +603,7 @@ __global__ __launch_bounds__(1024) void k_body(Geom g, Ctrl *ctrl,
       ctrl->err |= kErrBandEscape;
   }
   ctrl->sphere = b;
-  for (int j = 0; j < 6; j++) ctrl->box_band[j] = ctrl->box_fused[j];
+  for (int j = 0; j < 6; j++) ctrl->box_band[j] = box[j];
   int nb[6];
   band_box(b, g.nx, g.ny, g.nz, nb);
   for (int j = 0; j < 6; j++) ctrl->box_fused[j] = nb[j];
@@ -639,7 +612,6 @@ __global__ __launch_bounds__(1024) void k_body(Geom g, Ctrl *ctrl,
     ctrl->n_dead = 0;
     ctrl->n_solid = 0;
   }
-  *pcount = 0;
 }
 
 }  // namespace lbm

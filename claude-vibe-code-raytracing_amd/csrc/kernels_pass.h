@@ -143,17 +143,12 @@ __global__ __launch_bounds__(256) void k_retag(Geom g, int mode, const lbm_spher
   usz[i] = vz;
 }
 
-// Sorts one list ascending in a single workgroup (bitonic).  blockIdx.x selects
-// the fresh (0) or dead (1) list.  Lists up to kSortLds entries sort in LDS;
-// longer ones sort in place in global memory, padded with INT_MAX up to the next
-// power of two (the list capacity is a power of two).
+// Sorts a[0..count) ascending within one workgroup (bitonic).  Lists up to
+// kSortLds entries sort in the LDS buffer sh; longer ones sort in place in
+// global memory, padded with INT_MAX up to the next power of two (list
+// capacities are powers of two).
 constexpr int kSortLds = 8192;
-__global__ __launch_bounds__(1024) void k_sort_lists(int32_t *fresh, int32_t *dead, Ctrl *ctrl,
-                                                     int cap) {
-  __shared__ int32_t sh[kSortLds];
-  int32_t *a = blockIdx.x == 0 ? fresh : dead;
-  int count = blockIdx.x == 0 ? ctrl->n_fresh : ctrl->n_dead;
-  if (count > cap) count = cap;
+__device__ void sort_list_block(int32_t *a, int count, int32_t *sh) {
   if (count <= 1) return;
   int P = 1;
   while (P < count) P <<= 1;
@@ -187,6 +182,17 @@ __global__ __launch_bounds__(1024) void k_sort_lists(int32_t *fresh, int32_t *de
   }
   if (P <= kSortLds)
     for (int t = tid; t < count; t += blockDim.x) a[t] = sh[t];
+  __threadfence_block();
+  __syncthreads();
+}
+
+// Sorts the fresh (block 0) or dead (block 1) list of the last retag.
+__global__ __launch_bounds__(1024) void k_sort_lists(int32_t *fresh, int32_t *dead, Ctrl *ctrl,
+                                                     int cap) {
+  __shared__ int32_t sh[kSortLds];
+  int count = blockIdx.x == 0 ? ctrl->n_fresh : ctrl->n_dead;
+  if (count > cap) count = cap;
+  sort_list_block(blockIdx.x == 0 ? fresh : dead, count, sh);
 }
 
 __device__ __forceinline__ int lower_bound_i32(const int32_t *a, int n, int32_t v) {
@@ -207,20 +213,15 @@ __device__ __forceinline__ int lower_bound_i32(const int32_t *a, int n, int32_t 
 // all cells of a level are processed together, which reproduces the sequential
 // order exactly (a later fresh neighbour always has a higher level).
 //
-// phi_nb: phi read for neighbours; phi_fb: phiPrev read for the no-neighbour
-// fallback; phi_w1/phi_w2: the two writes of phi[i] = phiPrev[i] = p (they may
-// alias).  f and h receive the reseeded populations.
-__global__ __launch_bounds__(1024) void k_fresh(Geom g, double rhoG, double rhoL,
-                                                const int32_t *__restrict__ list, Ctrl *ctrl,
-                                                int cap, int *level_scratch,
-                                                const uint8_t *__restrict__ tag,
-                                                const float *phi_fb, float *phi_w1, float *phi_w2,
-                                                float *rho, float *ux, float *uy, float *uz,
-                                                float *f, float *h) {
+// phi_fb: phiPrev read for the no-neighbour fallback; phi_w1/phi_w2: the two
+// writes of phi[i] = phiPrev[i] = p (they may alias); neighbours read phi_w1.
+// f and h receive the reseeded populations.
+__device__ void fresh_block(Geom g, double rhoG, double rhoL, const int32_t *__restrict__ list,
+                            int count, int *level_scratch, const uint8_t *__restrict__ tag,
+                            const float *phi_fb, float *phi_w1, float *phi_w2, float *rho,
+                            float *ux, float *uy, float *uz, float *f, float *h) {
   __shared__ int sh_level[4096];
   __shared__ int sh_flag;
-  int count = ctrl->n_fresh;
-  if (count > cap) count = cap;
   if (count == 0) return;
   const int tid = threadIdx.x;
   int *lv = count <= 4096 ? sh_level : level_scratch;
@@ -309,6 +310,36 @@ __global__ __launch_bounds__(1024) void k_fresh(Geom g, double rhoG, double rhoL
     __threadfence();
     __syncthreads();
   }
+}
+
+__global__ __launch_bounds__(1024) void k_fresh(Geom g, double rhoG, double rhoL,
+                                                const int32_t *__restrict__ list, Ctrl *ctrl,
+                                                int cap, int *level_scratch,
+                                                const uint8_t *__restrict__ tag,
+                                                const float *phi_fb, float *phi_w1, float *phi_w2,
+                                                float *rho, float *ux, float *uy, float *uz,
+                                                float *f, float *h) {
+  int count = ctrl->n_fresh;
+  if (count > cap) count = cap;
+  fresh_block(g, rhoG, rhoL, list, count, level_scratch, tag, phi_fb, phi_w1, phi_w2, rho, ux, uy,
+              uz, f, h);
+}
+
+// Sorts both lists of a retag, then runs handleFreshCells: one launch for the
+// single-workgroup part of the fused step's band update.
+__global__ __launch_bounds__(1024) void k_sort_fresh(Geom g, double rhoG, double rhoL,
+                                                     int32_t *fresh, int32_t *dead, Ctrl *ctrl,
+                                                     int cap, int *level_scratch,
+                                                     const uint8_t *__restrict__ tag,
+                                                     float *phi, float *rho, float *ux, float *uy,
+                                                     float *uz, float *f, float *h) {
+  __shared__ int32_t sh[kSortLds];
+  int nf = ctrl->n_fresh, nd = ctrl->n_dead;
+  if (nf > cap) nf = cap;
+  if (nd > cap) nd = cap;
+  sort_list_block(fresh, nf, sh);
+  sort_list_block(dead, nd, sh);
+  fresh_block(g, rhoG, rhoL, fresh, nf, level_scratch, tag, phi, phi, phi, rho, ux, uy, uz, f, h);
 }
 
 // computePhiGradients (src/sim.js:176-199): all cells, clamped indices.

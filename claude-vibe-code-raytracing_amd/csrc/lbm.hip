@@ -876,6 +876,49 @@ extern "C" int lbm_kernel_time(lbm_sim *s, const char *name, double *total_ms, i
   return LBM_OK;
 }
 
+// ------------------------------------------------------------------ bandwidth probe
+
+__global__ __launch_bounds__(256) void k_copy_probe(const float4 *__restrict__ a,
+                                                    float4 *__restrict__ b, int64_t n4) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n4) b[i] = a[i];
+}
+
+extern "C" int lbm_probe_copy_bandwidth(int64_t bytes, int reps, double *gbps) {
+  if (bytes < 16 || reps < 1 || !gbps) return LBM_ERR_ARG;
+  const int64_t n4 = bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = LBM_ERR_HIP;
+  float ms = 0;
+  if (hipMalloc(&a, n4 * 16) != hipSuccess || hipMalloc(&b, n4 * 16) != hipSuccess) {
+    rc = LBM_ERR_OOM;
+    goto done;
+  }
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) goto done;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) goto done;
+  if (hipMemsetAsync(a, 0, n4 * 16, st) != hipSuccess) goto done;
+  {
+    const int64_t blocks = (n4 + 255) / 256;
+    k_copy_probe<<<blocks, 256, 0, st>>>(a, b, n4);  // warm-up
+    hipEventRecord(e0, st);
+    for (int r = 0; r < reps; r++) k_copy_probe<<<blocks, 256, 0, st>>>(a, b, n4);
+    hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess) goto done;
+    hipEventElapsedTime(&ms, e0, e1);
+    *gbps = 2.0 * (double)n4 * 16 * reps / (ms * 1e-3) / 1e9;
+    rc = LBM_OK;
+  }
+done:
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (st) hipStreamDestroy(st);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  return rc;
+}
+
 // ------------------------------------------------------------------ coupled step
 
 #include "coupled.inc"

@@ -116,6 +116,7 @@ def lib():
         "lbm_kernel_time": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double),
                                       C.POINTER(C.c_int64)]),
         "lbm_version": (C.c_char_p, []),
+        "lbm_probe_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -131,6 +132,15 @@ def make_sphere(nx, ny, nz, cx=float("nan"), cy=float("nan"), cz=float("nan"),
     b = Sphere()
     lib().lbm_sphere_init(C.byref(b), nx, ny, nz, cx, cy, cz, r, density)
     return b
+
+
+def copy_bandwidth(nbytes=1 << 31, reps=10):
+    """Measured read+write GB/s of a plain device copy of nbytes."""
+    g = C.c_double(0)
+    rc = lib().lbm_probe_copy_bandwidth(int(nbytes), int(reps), C.byref(g))
+    if rc != 0:
+        raise LbmError(f"lbm_probe_copy_bandwidth = {rc}")
+    return g.value
 
 
 def sphere_step(b, dt, gravity):

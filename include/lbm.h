@@ -242,6 +242,13 @@ int lbm_set_timing(lbm_sim *sim, int on);
  * "stream_macro". */
 int lbm_kernel_time(lbm_sim *sim, const char *name, double *total_ms, int64_t *launches);
 
+/* Measured device bandwidth of a plain copy: copies `bytes` (rounded down to 16)
+ * between two device buffers `reps` times with one float4 per thread and
+ * returns the read+write rate in GB/s in *gbps.  No reference counterpart: it
+ * is the measured ceiling the roofline figures are compared with, and the
+ * kernel of known size that calibrates hardware traffic counters. */
+int lbm_probe_copy_bandwidth(int64_t bytes, int reps, double *gbps);
+
 /* Library version string. */
 const char *lbm_version(void);
 
