@@ -41,13 +41,19 @@ constexpr int kErrListOverflow = 1;
 constexpr int kErrBandEscape = 2;
 constexpr int kErrFreshCapacity = 4;
 
-// Tiles of 64 x 4 x 1 cells (one wave per x-row segment) launched on a 1-D grid
-// in XCD-aware order.  The hardware hands consecutive workgroups to the eight
-// XCDs in turn, and each XCD has its own L2; mapping physical workgroup p to
-// tile (p % 8) * (T8 / 8) + p / 8 gives every XCD a contiguous run of tiles (a
-// slab of z-planes), so tiles that share halo rows and cache lines run on the
-// same L2.
+// Tiles of TX x TY x 1 cells (TX * TY = 256, four waves of 64 consecutive x)
+// launched on a 1-D grid in XCD-aware order.
+//  * Tile width: the widest of 256, 128, 64 cells that divides the row padded
+//    to 64.  Each population plane is then read and written in runs of TX
+//    consecutive cells per workgroup; with 34 planes in flight, 1 KB runs keep
+//    HBM pages open far better than 256 B runs (measured on MI355X: 4.4 vs
+//    3.9 TB/s for the 34-plane pull pattern).
+//  * Order: the hardware hands consecutive workgroups to the eight XCDs in
+//    turn, and each XCD has its own L2; mapping physical workgroup p to tile
+//    (p % 8) * (T8 / 8) + p / 8 gives every XCD a contiguous run of tiles (a
+//    slab of z-planes), so tiles that share halo rows run on the same L2.
 struct TileMap {
+  int TX, TY;      // tile size in cells (TX * TY = 256)
   int tx, ty, tz;  // tiles per axis
   int T;           // tx * ty * tz
   int T8;          // T rounded up to a multiple of 8 (the launched grid)
@@ -55,15 +61,19 @@ struct TileMap {
 
 inline TileMap make_tile_map(int nx, int ny, int nz) {
   TileMap m;
-  m.tx = (nx + 63) / 64;
-  m.ty = (ny + 3) / 4;
+  const int row = (nx + 63) / 64 * 64;
+  m.TX = row % 256 == 0 ? 256 : row % 128 == 0 ? 128 : 64;
+  m.TY = 256 / m.TX;
+  m.tx = row / m.TX;
+  m.ty = (ny + m.TY - 1) / m.TY;
   m.tz = nz;
   m.T = m.tx * m.ty * m.tz;
   m.T8 = (m.T + 7) / 8 * 8;
   return m;
 }
 
-// Tile of this workgroup; false for the padding workgroups past T.
+// Tile of this workgroup and the cell of this thread; false for the padding
+// workgroups past T.  Workgroups are 64 x 4 threads (threadIdx.y is the wave).
 __device__ __forceinline__ bool map_tile(const TileMap &m, int &tile, int &bx, int &by, int &bz) {
   const int p = blockIdx.x;
   tile = (p & 7) * (m.T8 >> 3) + (p >> 3);
@@ -72,6 +82,14 @@ __device__ __forceinline__ bool map_tile(const TileMap &m, int &tile, int &bx, i
   by = (tile / m.tx) % m.ty;
   bz = tile / (m.tx * m.ty);
   return true;
+}
+
+__device__ __forceinline__ void tile_cell(const TileMap &m, int bx, int by, int bz, int &x, int &y,
+                                          int &z) {
+  const int t = threadIdx.y * 64 + threadIdx.x;
+  x = bx * m.TX + t % m.TX;
+  y = by * m.TY + t / m.TX;
+  z = bz;
 }
 
 __device__ __forceinline__ int64_t cell_index(const Geom &g, int x, int y, int z) {

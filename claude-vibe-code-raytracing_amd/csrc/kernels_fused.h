@@ -66,11 +66,9 @@ __global__ __launch_bounds__(256) void k_phase_stream(Geom g, TileMap tm,
                                                       const uint8_t *__restrict__ tag,
                                                       const float *__restrict__ phi,
                                                       float *__restrict__ phi_next) {
-  int tile, bx, by, bz;
+  int tile, bx, by, bz, x, y, z;
   if (!map_tile(tm, tile, bx, by, bz)) return;
-  const int x = bx * 64 + threadIdx.x;
-  const int y = by * 4 + threadIdx.y;
-  const int z = bz;
+  tile_cell(tm, bx, by, bz, x, y, z);
   if (x >= g.nx || y >= g.ny) return;
   const int64_t i = cell_index(g, x, y, z);
   if (tag[i] != TAG_FLUID || !interior(g, x, y, z)) {
@@ -303,7 +301,7 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Link momentum of one 64x4 tile (6 doubles), summed in a fixed order (butterfly
+// Link momentum of one tile (6 doubles), summed in a fixed order (butterfly
 // within each wave, waves in order) and stored at partials[tile].  Every tile
 // that intersects the band box stores its sum, zero or not, so the body update
 // can read exactly the band's tiles without a list.
@@ -342,19 +340,18 @@ __global__ __launch_bounds__(256) void k_stream_collide(
     const float *__restrict__ usx, const float *__restrict__ usy, const float *__restrict__ usz,
     const float *__restrict__ bfx, const float *__restrict__ bfy, const float *__restrict__ bfz,
     int use_bf, const Ctrl *__restrict__ ctrl, int has_solid, double *__restrict__ partials) {
-  int tile, bx, by, bz;
+  int tile, bx, by, bz, x, y, z;
   if (!map_tile(tm, tile, bx, by, bz)) return;
-  const int x = bx * 64 + threadIdx.x;
-  const int y = by * 4 + threadIdx.y;
-  const int z = bz;
+  tile_cell(tm, bx, by, bz, x, y, z);
   double mom[6] = {0, 0, 0, 0, 0, 0};
   bool linked = false;
   int box[6];
 #pragma unroll
   for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
   // Link momentum can only arise next to the sphere, which lies inside the band.
-  const bool tile_in_band = bx * 64 <= box[1] && bx * 64 + 63 >= box[0] && by * 4 <= box[3] &&
-                            by * 4 + 3 >= box[2] && bz >= box[4] && bz <= box[5];
+  const bool tile_in_band = bx * tm.TX <= box[1] && bx * tm.TX + tm.TX - 1 >= box[0] &&
+                            by * tm.TY <= box[3] && by * tm.TY + tm.TY - 1 >= box[2] &&
+                            bz >= box[4] && bz <= box[5];
   const bool exchange = has_solid != 0 && tile_in_band;
   double cx = 0, cy = 0, cz = 0;
   if (exchange) {
@@ -527,8 +524,8 @@ __global__ __launch_bounds__(1024) void k_body(Geom g, TileMap tm, Ctrl *ctrl,
   int box[6];
 #pragma unroll
   for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
-  const int bx0 = box[0] >> 6, nbx = (box[1] >> 6) - bx0 + 1;
-  const int by0 = box[2] >> 2, nby = (box[3] >> 2) - by0 + 1;
+  const int bx0 = box[0] / tm.TX, nbx = box[1] / tm.TX - bx0 + 1;
+  const int by0 = box[2] / tm.TY, nby = box[3] / tm.TY - by0 + 1;
   const int bz0 = box[4], nbz = box[5] - box[4] + 1;
   const int ntiles = nbx * nby * nbz;
   double acc[12];
