@@ -8,10 +8,15 @@ absorbDeadCells; sphere step.  Prints one JSON line.
 
 MLUPS counts all nx*ny*nz cells, walls included.  The roofline object is for
 the dominant kernel (stream + macro + next collide), timed with HIP events on
-the library's stream inside the timed region; its algorithmic traffic is 297 B
-per cell (27+7 populations read and written, phi(t) and phi(t+1) read, rho and
-u written, tag read).  The CPU baseline is the oracle (a single-threaded C
-restatement of the reference) on the same scene for a few steps."""
+the library's stream inside the timed region; its algorithmic traffic is 281 B
+per cell (27+7 populations read and written, phi(t), phi(t+1) and the tag
+read; bulk cells store rho and u only at the last step of a call).  The
+whole-step fraction uses 297 B per cell (the same plus the phi write of the
+phase pass and the rho/u writes).  `traffic` is the measured HBM bytes per
+launch from profiles/stream_collide_256cube_traffic.json (rocprofv3
+FETCH_SIZE/WRITE_SIZE, calibrated on a copy kernel), for the 256^3 scene only.
+The CPU baseline is the oracle (a single-threaded C restatement of the
+reference) on the same scene for a few steps."""
 import argparse
 import json
 import os
@@ -28,7 +33,20 @@ import lbm  # noqa: E402
 import scenegen  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_LU = {"stream_collide": 297, "phase_stream": 33, "stream_macro": 297}
+BYTES_PER_LU = {"stream_collide": 281, "phase_stream": 33, "stream_macro": 297}
+STEP_BYTES_PER_LU = 297
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "stream_collide_256cube_traffic.json")
+
+
+def measured_traffic(n):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    profile, when it was taken on the same grid."""
+    try:
+        with open(TRAFFIC_JSON) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    return t["traffic_bytes"] if t.get("cells") == n ** 3 else None
 METRIC = "coupled MLUPS + %HBM peak at 256^3; served scenes MLUPS, p99 frame latency"
 
 
@@ -135,10 +153,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None, "kernel": "k_stream_collide",
+                     "traffic": measured_traffic(n), "kernel": "k_stream_collide",
                      "bytes_per_lu": BYTES_PER_LU["stream_collide"],
                      "avg_ms": round(avg_s * 1e3, 4)},
-        "whole_step_hbm_frac": round(297 * mlups * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
+        "whole_step_hbm_frac": round(STEP_BYTES_PER_LU * mlups * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
         "copy_probe_gbps": round(copy_gbps, 1),
         "kernel_ms": {k: {"total": round(v[0], 3), "launches": v[1]} for k, v in kt.items()},
         "finite": finite,
