@@ -6,15 +6,21 @@
 // only global dependency, the sphere position, which needs the momentum sum of
 // stream(t) before retag(t+1) can run:
 //
-//   k_phase_stream(t)     phi(t+1) = clamp(sum of pulled h) for every cell
-//   k_stream_collide(t)   stream(t) + macro(t) for every fluid cell, then
+//   km_phase_stream(t)    phi(t+1) = clamp(sum of pulled h) for every cell
+//   km_stream_collide(t)  stream(t) + macro(t) for every fluid cell, then
 //                         collide(t+1) for every fluid cell outside the band;
 //                         band cells keep their post-stream populations
-//   k_body(t)             momentum sums in a fixed order, dead-cell impulses,
+//   km_body(t)            momentum sums in a fixed order, dead-cell impulses,
 //                         sphere step, next band boxes
-//   k_retag(t+1, band)    retag inside the band box only
-//   k_fresh(t+1)          fresh cells, in the reference's order
-//   k_collide_region(t+1, band)  gradients + collide for band cells
+//   km_retag_band(t+1)    retag inside the band box only
+//   km_sort_fresh(t+1)    fresh cells, in the reference's order
+//   km_collide(t+1, band) gradients + collide for band cells
+//
+// Every kernel works on a set of scenes (independent simulations of any size):
+// a Scene descriptor in device memory holds a scene's geometry, parameters,
+// buffers and buffer parities, and a launch covers the blocks of all scenes,
+// each block finding its scene in a prefix table of block offsets.  One scene
+// is the single-simulation case; many scenes are the batched form.
 //
 // The band box holds every cell whose tag can change at t+1, every fresh cell
 // and its face neighbours (the only cells whose collide(t+1) reads values the
@@ -27,6 +33,7 @@
 #pragma once
 
 #include "device_common.h"
+#include "kernels_pass.h"
 
 namespace lbm {
 
@@ -57,44 +64,6 @@ __host__ __device__ inline void band_box(const lbm_sphere &b, int nx, int ny, in
 
 __device__ __forceinline__ bool in_box(const int box[6], int x, int y, int z) {
   return x >= box[0] && x <= box[1] && y >= box[2] && y <= box[3] && z >= box[4] && z <= box[5];
-}
-
-// phi(t+1) for every cell: fluid cells sum their pulled h (src/sim.js:376-397,
-// 549-552); other cells keep phi (the reference never updates them).
-__global__ __launch_bounds__(256) void k_phase_stream(Geom g, TileMap tm,
-                                                      const float *__restrict__ h2,
-                                                      const uint8_t *__restrict__ tag,
-                                                      const float *__restrict__ phi,
-                                                      float *__restrict__ phi_next) {
-  int tile, bx, by, bz, x, y, z;
-  if (!map_tile(tm, tile, bx, by, bz)) return;
-  tile_cell(tm, bx, by, bz, x, y, z);
-  if (x >= g.nx || y >= g.ny) return;
-  const int64_t i = cell_index(g, x, y, z);
-  if (tag[i] != TAG_FLUID || !interior(g, x, y, z)) {
-    phi_next[i] = phi[i];
-    return;
-  }
-  // All source tags first, then every load with its address already chosen, so
-  // the seven loads are in flight together.
-  uint32_t nonfluid = 0;
-#pragma unroll
-  for (int k = 1; k < Q7; k++) {
-    const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
-    nonfluid |= (uint32_t)(tag[s] != TAG_FLUID) << k;
-  }
-  float hv[Q7];
-#pragma unroll
-  for (int k = 0; k < Q7; k++) {
-    const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
-    const int64_t a = ((nonfluid >> k) & 1u) ? opp7(k) * g.S + i : k * g.S + s;
-    hv[k] = h2[a];
-  }
-  double phiSum = 0;
-#pragma unroll
-  for (int k = 0; k < Q7; k++) phiSum += (double)hv[k];
-  const double p = phiSum < 0 ? 0 : phiSum > 1 ? 1 : phiSum;
-  phi_next[i] = (float)p;
 }
 
 // Gradient and Laplacian of a phi array at interior cell i (src/sim.js:186-195),
@@ -323,36 +292,253 @@ __device__ __forceinline__ void tile_momentum(double mom[6], double *partials, i
   }
 }
 
-// stream(t) + macro(t) for every fluid cell; with COLLIDE, also collide(t+1)
-// for fluid cells outside the band.  Without COLLIDE (the last step of a call)
-// the post-stream populations are stored for every cell, which leaves the state
-// exactly as the reference's step does, and phi(t+1) is written here.
-//
-// f2/h2: post-collision populations of step t.  fo/ho: output planes.
-// phi: phi(t) after fresh(t).  phi_next: phi(t+1) (read with COLLIDE, written
-// without).
-template <bool COLLIDE>
-__global__ __launch_bounds__(256) void k_stream_collide(
-    Geom g, TileMap tm, Derived d, const float *__restrict__ f2, const float *__restrict__ h2,
-    float *__restrict__ fo, float *__restrict__ ho, const float *__restrict__ phi,
-    float *__restrict__ phi_next, float *__restrict__ rho, float *__restrict__ ux,
-    float *__restrict__ uy, float *__restrict__ uz, const uint8_t *__restrict__ tag,
-    const float *__restrict__ usx, const float *__restrict__ usy, const float *__restrict__ usz,
-    const float *__restrict__ bfx, const float *__restrict__ bfy, const float *__restrict__ bfz,
-    int use_bf, const Ctrl *__restrict__ ctrl, int has_solid, double *__restrict__ partials) {
+
+// ---------------------------------------------------------------- scenes
+
+// One simulation of a scene set.  Buffer pairs: at a steady step of global
+// parity q the post-collision populations are f[ef ^ q] (written by the
+// previous step) and the step writes f[ef ^ q ^ 1]; phi(t) is phi[ep ^ q] and
+// phi(t+1) phi[ep ^ q ^ 1].
+struct Scene {
+  Geom g;
+  TileMap tm;
+  Derived d;
+  float *f[2], *h[2], *phi[2];
+  float *rho, *ux, *uy, *uz;
+  uint8_t *tag, *tagPrev;
+  float *usx, *usy, *usz;
+  float *bfx, *bfy, *bfz;
+  int32_t *fresh, *dead;
+  int *level_scratch;
+  Ctrl *ctrl;
+  double *partials;
+  int list_cap;
+  int ef, ep;               // 0 or 1
+  double rhoL;              // liquid density, as set (fresh cells use rhoL - rhoG)
+  int has_solid;
+  int use_bf;               // external force armed for the first step of the call
+  int cgx, cgy;             // full-grid blocks of 64x4 cells: cgx * cgy * nz
+  int bgx, bgy, bgz;        // band blocks of 64x4 cells (covers the largest band box)
+};
+
+// Scene of block b: off[s] <= b < off[s + 1].
+__device__ __forceinline__ int find_scene(const int *__restrict__ off, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Tile of a block of a tile-mapped launch (TileMap order, offsets multiples of 8
+// so the XCD-aware order holds inside every scene).
+__device__ __forceinline__ bool scene_tile(const Scene &sc, int lb, int &tile, int &bx, int &by,
+                                           int &bz) {
+  const TileMap &m = sc.tm;
+  tile = (lb & 7) * (m.T8 >> 3) + (lb >> 3);
+  if (tile >= m.T) return false;
+  bx = tile % m.tx;
+  by = (tile / m.tx) % m.ty;
+  bz = tile / (m.tx * m.ty);
+  return true;
+}
+
+// ---------------------------------------------------------------- retag
+
+// retagCells over the whole grid (first step of a call) or over the band box
+// (later steps).  No phi snapshot: in the fused form phiPrev is the other phi
+// buffer.
+template <bool BAND>
+__global__ __launch_bounds__(256) void km_retag(const Scene *__restrict__ scenes,
+                                                const int *__restrict__ off, int nscenes) {
+  const int si = find_scene(off, nscenes, blockIdx.x);
+  const Scene &sc = scenes[si];
+  const int lb = blockIdx.x - off[si];
+  int x, y, z;
+  if (BAND) {
+    const int *box = sc.ctrl->box_band;
+    x = box[0] + (lb % sc.bgx) * 64 + threadIdx.x;
+    y = box[2] + ((lb / sc.bgx) % sc.bgy) * 4 + threadIdx.y;
+    z = box[4] + lb / (sc.bgx * sc.bgy);
+    if (x > box[1] || y > box[3] || z > box[5]) return;
+  } else {
+    x = (lb % sc.cgx) * 64 + threadIdx.x;
+    y = ((lb / sc.cgx) % sc.cgy) * 4 + threadIdx.y;
+    z = lb / (sc.cgx * sc.cgy);
+    if (x >= sc.g.nx || y >= sc.g.ny) return;
+  }
+  lbm_sphere b{};
+  if (sc.has_solid) b = sc.ctrl->sphere;
+  retag_cell(sc.g, sc.has_solid ? 1 : 0, b, nullptr, nullptr, sc.tag, sc.tagPrev, nullptr,
+             nullptr, 0, sc.usx, sc.usy, sc.usz, sc.ctrl, sc.fresh, sc.dead, sc.list_cap, x, y, z);
+}
+
+// Sorts both lists of the scene's last retag, then runs handleFreshCells; one
+// workgroup per scene.  Upcoming step p: phi is phi[ep ^ p]; the post-stream
+// populations of fresh cells go to the reference's f (f[ef ^ 1]) before the
+// first step of a call, and in place into f[ef ^ p] for band updates.
+__global__ __launch_bounds__(1024) void km_sort_fresh(const Scene *__restrict__ scenes, int p,
+                                                      int prologue) {
+  __shared__ int32_t sh[kSortLds];
+  const Scene &sc = scenes[blockIdx.x];
+  int nf = sc.ctrl->n_fresh, nd = sc.ctrl->n_dead;
+  if (nf > sc.list_cap) nf = sc.list_cap;
+  if (nd > sc.list_cap) nd = sc.list_cap;
+  sort_list_block(sc.fresh, nf, sh);
+  sort_list_block(sc.dead, nd, sh);
+  p &= 1;  // buffer parity of the upcoming step
+  const int fi = prologue ? (sc.ef ^ 1) : (sc.ef ^ p);
+  float *phi = sc.phi[sc.ep ^ p];
+  fresh_block(sc.g, sc.d.rhoG, sc.rhoL, sc.fresh, nf, sc.level_scratch, sc.tag, phi, phi, phi,
+              sc.rho, sc.ux, sc.uy, sc.uz, sc.f[fi], sc.h[fi]);
+}
+
+// ---------------------------------------------------------------- collide
+
+// Gradients + collide for the fluid cells of the whole grid (first step of a
+// call: post-stream populations in f[ef ^ 1] -> post-collision f[ef], the
+// external force if armed) or of the band box after step p - 1 (in place in
+// f[ef ^ p]).
+template <bool BAND>
+__global__ __launch_bounds__(256) void km_collide(const Scene *__restrict__ scenes,
+                                                  const int *__restrict__ off, int nscenes,
+                                                  int p) {
+  const int si = find_scene(off, nscenes, blockIdx.x);
+  const Scene &sc = scenes[si];
+  const Geom &g = sc.g;
+  const int lb = blockIdx.x - off[si];
+  int x, y, z;
+  if (BAND) {
+    const int *box = sc.ctrl->box_band;
+    x = box[0] + (lb % sc.bgx) * 64 + threadIdx.x;
+    y = box[2] + ((lb / sc.bgx) % sc.bgy) * 4 + threadIdx.y;
+    z = box[4] + lb / (sc.bgx * sc.bgy);
+    if (x > box[1] || y > box[3] || z > box[5]) return;
+  } else {
+    x = (lb % sc.cgx) * 64 + threadIdx.x;
+    y = ((lb / sc.cgx) % sc.cgy) * 4 + threadIdx.y;
+    z = lb / (sc.cgx * sc.cgy);
+    if (x >= g.nx || y >= g.ny) return;
+  }
+  if (!interior(g, x, y, z)) return;
+  const int64_t i = cell_index(g, x, y, z);
+  if (sc.tag[i] != TAG_FLUID) return;
+  p &= 1;  // buffer parity of the upcoming step
+  const float *fi = BAND ? sc.f[sc.ef ^ p] : sc.f[sc.ef ^ 1];
+  float *fo = BAND ? sc.f[sc.ef ^ p] : sc.f[sc.ef];
+  const float *hi = BAND ? sc.h[sc.ef ^ p] : sc.h[sc.ef ^ 1];
+  float *ho = BAND ? sc.h[sc.ef ^ p] : sc.h[sc.ef];
+  const float *phi = sc.phi[sc.ep ^ p];
+  const double pc = phi[i];
+  float gx1, gy1, gz1, lap1;
+  phi_grad(g, phi, i, pc, gx1, gy1, gz1, lap1);
+  const double uxe = sc.ux[i], uye = sc.uy[i], uze = sc.uz[i];
+  float hin[Q7], hc[Q7];
+#pragma unroll
+  for (int k = 0; k < Q7; k++) hin[k] = hi[k * g.S + i];
+  collide_phase_cell(sc.d, pc, uxe, uye, uze, (double)gx1, (double)gy1, (double)gz1, hin, hc);
+#pragma unroll
+  for (int k = 0; k < Q7; k++) ho[k * g.S + i] = hc[k];
+  float fin[Q27], fc[Q27];
+#pragma unroll
+  for (int k = 0; k < Q27; k++) fin[k] = fi[k * g.S + i];
+  const bool bf = !BAND && sc.use_bf;
+  const double bx = bf ? (double)sc.bfx[i] : 0.0;
+  const double by = bf ? (double)sc.bfy[i] : 0.0;
+  const double bz = bf ? (double)sc.bfz[i] : 0.0;
+  collide_hydro_cell(sc.d, (double)sc.rho[i], pc, (double)lap1, (double)gx1, (double)gy1,
+                     (double)gz1, bx, by, bz, uxe, uye, uze, fin, fc);
+#pragma unroll
+  for (int k = 0; k < Q27; k++) fo[k * g.S + i] = fc[k];
+}
+
+// ---------------------------------------------------------------- phase stream
+
+// phi(t+1) for every cell of step q: fluid cells sum their pulled h
+// (src/sim.js:376-397, 549-552); other cells keep phi (the reference never
+// updates them).  Source tags are read first and every load address is chosen
+// without branching, so the seven loads are in flight together.
+__global__ __launch_bounds__(256) void km_phase_stream(const Scene *__restrict__ scenes,
+                                                       const int *__restrict__ off, int nscenes,
+                                                       int q) {
+  const int si = find_scene(off, nscenes, blockIdx.x);
+  const Scene &sc = scenes[si];
+  const Geom &g = sc.g;
   int tile, bx, by, bz, x, y, z;
-  if (!map_tile(tm, tile, bx, by, bz)) return;
-  tile_cell(tm, bx, by, bz, x, y, z);
+  if (!scene_tile(sc, blockIdx.x - off[si], tile, bx, by, bz)) return;
+  tile_cell(sc.tm, bx, by, bz, x, y, z);
+  if (x >= g.nx || y >= g.ny) return;
+  q &= 1;
+  const float *h2 = sc.h[sc.ef ^ q];
+  const float *phi = sc.phi[sc.ep ^ q];
+  float *phi_next = sc.phi[sc.ep ^ q ^ 1];
+  const uint8_t *tag = sc.tag;
+  const int64_t i = cell_index(g, x, y, z);
+  if (tag[i] != TAG_FLUID || !interior(g, x, y, z)) {
+    phi_next[i] = phi[i];
+    return;
+  }
+  uint32_t nonfluid = 0;
+#pragma unroll
+  for (int k = 1; k < Q7; k++) {
+    const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
+    nonfluid |= (uint32_t)(tag[s] != TAG_FLUID) << k;
+  }
+  float hv[Q7];
+#pragma unroll
+  for (int k = 0; k < Q7; k++) {
+    const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
+    const int64_t a = ((nonfluid >> k) & 1u) ? opp7(k) * g.S + i : k * g.S + s;
+    hv[k] = h2[a];
+  }
+  double phiSum = 0;
+#pragma unroll
+  for (int k = 0; k < Q7; k++) phiSum += (double)hv[k];
+  const double pp = phiSum < 0 ? 0 : phiSum > 1 ? 1 : phiSum;
+  phi_next[i] = (float)pp;
+}
+
+// ---------------------------------------------------------------- stream + collide
+
+// stream(t) + macro(t) for every fluid cell of step q; with COLLIDE, also
+// collide(t+1) for fluid cells outside the band.  Without COLLIDE (the last
+// step of a call) the post-stream populations are stored for every cell, which
+// leaves the state exactly as the reference's step does, and phi(t+1) is
+// written here.  first: the first step of the call (the armed external force
+// applies to its macroscopic update).
+template <bool COLLIDE>
+__global__ __launch_bounds__(256) void km_stream_collide(const Scene *__restrict__ scenes,
+                                                         const int *__restrict__ off, int nscenes,
+                                                         int q, int first) {
+  const int si = find_scene(off, nscenes, blockIdx.x);
+  const Scene &sc = scenes[si];
+  const Geom &g = sc.g;
+  const Derived &d = sc.d;
+  int tile, bx, by, bz, x, y, z;
+  if (!scene_tile(sc, blockIdx.x - off[si], tile, bx, by, bz)) return;
+  tile_cell(sc.tm, bx, by, bz, x, y, z);
+  q &= 1;
+  const float *__restrict__ f2 = sc.f[sc.ef ^ q];
+  const float *__restrict__ h2 = sc.h[sc.ef ^ q];
+  float *__restrict__ fo = sc.f[sc.ef ^ q ^ 1];
+  float *__restrict__ ho = sc.h[sc.ef ^ q ^ 1];
+  const float *__restrict__ phi = sc.phi[sc.ep ^ q];
+  float *__restrict__ phi_next = sc.phi[sc.ep ^ q ^ 1];
+  const uint8_t *__restrict__ tag = sc.tag;
+  const Ctrl *__restrict__ ctrl = sc.ctrl;
   double mom[6] = {0, 0, 0, 0, 0, 0};
   bool linked = false;
   int box[6];
 #pragma unroll
   for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
   // Link momentum can only arise next to the sphere, which lies inside the band.
-  const bool tile_in_band = bx * tm.TX <= box[1] && bx * tm.TX + tm.TX - 1 >= box[0] &&
-                            by * tm.TY <= box[3] && by * tm.TY + tm.TY - 1 >= box[2] &&
+  const bool tile_in_band = bx * sc.tm.TX <= box[1] && bx * sc.tm.TX + sc.tm.TX - 1 >= box[0] &&
+                            by * sc.tm.TY <= box[3] && by * sc.tm.TY + sc.tm.TY - 1 >= box[2] &&
                             bz >= box[4] && bz <= box[5];
-  const bool exchange = has_solid != 0 && tile_in_band;
+  const bool exchange = sc.has_solid != 0 && tile_in_band;
+  const int use_bf = first && sc.use_bf;
   double cx = 0, cy = 0, cz = 0;
   if (exchange) {
     cx = ctrl->sphere.cx;
@@ -372,8 +558,8 @@ __global__ __launch_bounds__(256) void k_stream_collide(
       float fp[Q27];
       double pSumF, mx, my, mz;
       uint32_t nonfluid;
-      pull27(g, i, f2, tag, usx, usy, usz, rhoPhaseOld, exchange, x, y, z, cx, cy, cz, mom, linked,
-             fp, pSumF, mx, my, mz, nonfluid);
+      pull27(g, i, f2, tag, sc.usx, sc.usy, sc.usz, rhoPhaseOld, exchange, x, y, z, cx, cy, cz,
+             mom, linked, fp, pSumF, mx, my, mz, nonfluid);
       const double rhoN = pSumF > 0.01 ? pSumF : 0.01;
       // D3Q7 directions 1..6 are D3Q27 directions 1..6, so the same source mask
       // selects their bounce-back.
@@ -390,12 +576,12 @@ __global__ __launch_bounds__(256) void k_stream_collide(
       const double p = phiSum < 0 ? 0 : phiSum > 1 ? 1 : phiSum;
       const double rhoPhaseNew = d.rhoG + p * d.dRho;
       const double mu = 4 * d.beta * p * (p - 1) * (2 * p - 1) - d.kappa * (double)lapo;
-      const double bx = use_bf ? (double)bfx[i] : 0.0;
-      const double by = use_bf ? (double)bfy[i] : 0.0;
-      const double bz = use_bf ? (double)bfz[i] : 0.0;
-      const double fx = bx + mu * (double)gxo;
-      const double fy = by + mu * (double)gyo + rhoPhaseNew * d.gravity;
-      const double fz = bz + mu * (double)gzo;
+      const double bxf = use_bf ? (double)sc.bfx[i] : 0.0;
+      const double byf = use_bf ? (double)sc.bfy[i] : 0.0;
+      const double bzf = use_bf ? (double)sc.bfz[i] : 0.0;
+      const double fx = bxf + mu * (double)gxo;
+      const double fy = byf + mu * (double)gyo + rhoPhaseNew * d.gravity;
+      const double fz = bzf + mu * (double)gzo;
       const double invRho = 1 / rhoPhaseNew;
       double uxn = (mx + 0.5 * fx) * invRho;
       double uyn = (my + 0.5 * fy) * invRho;
@@ -403,29 +589,29 @@ __global__ __launch_bounds__(256) void k_stream_collide(
       const double u2 = uxn * uxn + uyn * uyn + uzn * uzn;
       const double UMAX = 0.2, UMAX2 = UMAX * UMAX;
       if (u2 > UMAX2) {
-        const double sc = UMAX / sqrt(u2);
-        uxn *= sc;
-        uyn *= sc;
-        uzn *= sc;
+        const double scl = UMAX / sqrt(u2);
+        uxn *= scl;
+        uyn *= scl;
+        uzn *= scl;
       }
       const float rhof = (float)rhoN, phif = (float)p;
       const float uxf = (float)uxn, uyf = (float)uyn, uzf = (float)uzn;
       const bool band = in_box(box, x, y, z);
-      // rho and u are read before the end of the call only inside the band (by the
-      // fresh-cell pass, the dead-cell impulses and the band collide); the last
-      // step of a call stores them everywhere.
+      // rho and u are read before the end of the call only inside the band (by
+      // the fresh-cell pass, the dead-cell impulses and the band collide); the
+      // last step of a call stores them everywhere.
       if (!COLLIDE || band) {
-        rho[i] = rhof;
-        ux[i] = uxf;
-        uy[i] = uyf;
-        uz[i] = uzf;
+        sc.rho[i] = rhof;
+        sc.ux[i] = uxf;
+        sc.uy[i] = uyf;
+        sc.uz[i] = uzf;
       }
       bool store_stream = true;
       if (COLLIDE) {
         if (!band) {
           store_stream = false;
-          // collide(t+1) with the new macroscopic values (rounded to float as the
-          // reference stores them) and the gradients of phi(t+1).
+          // collide(t+1) with the new macroscopic values (rounded to float as
+          // the reference stores them) and the gradients of phi(t+1).
           float gx1, gy1, gz1, lap1;
           phi_grad(g, phi_next, i, (double)phif, gx1, gy1, gz1, lap1);
           float hc[Q7];
@@ -434,9 +620,9 @@ __global__ __launch_bounds__(256) void k_stream_collide(
 #pragma unroll
           for (int k = 0; k < Q7; k++) ho[k * g.S + i] = hc[k];
           float fc[Q27];
-          collide_hydro_cell(d, (double)rhof, (double)phif, (double)lap1, (double)gx1, (double)gy1,
-                             (double)gz1, 0.0, 0.0, 0.0, (double)uxf, (double)uyf, (double)uzf, fp,
-                             fc);
+          collide_hydro_cell(d, (double)rhof, (double)phif, (double)lap1, (double)gx1,
+                             (double)gy1, (double)gz1, 0.0, 0.0, 0.0, (double)uxf, (double)uyf,
+                             (double)uzf, fp, fc);
 #pragma unroll
           for (int k = 0; k < Q27; k++) fo[k * g.S + i] = fc[k];
         }
@@ -452,75 +638,43 @@ __global__ __launch_bounds__(256) void k_stream_collide(
     }
   }
   (void)linked;
-  if (exchange) tile_momentum(mom, partials, tile);
+  if (exchange) tile_momentum(mom, sc.partials, tile);
 }
 
-// Gradients + collide for the fluid cells of a box, reading post-stream
-// populations and writing post-collision ones (in place when fi == fo).  Used for
-// the whole grid at the first step of a call and for the band afterwards.
-__global__ __launch_bounds__(256) void k_collide_region(
-    Geom g, Derived d, const float *fi, float *fo, const float *hi, float *ho,
-    const float *__restrict__ phi, const float *__restrict__ rho, const float *__restrict__ ux,
-    const float *__restrict__ uy, const float *__restrict__ uz, const uint8_t *__restrict__ tag,
-    const float *__restrict__ bfx, const float *__restrict__ bfy, const float *__restrict__ bfz,
-    int use_bf, const Ctrl *__restrict__ ctrl, int use_ctrl_box) {
-  int box[6] = {0, g.nx - 1, 0, g.ny - 1, 0, g.nz - 1};
-  if (use_ctrl_box) {
-#pragma unroll
-    for (int j = 0; j < 6; j++) box[j] = ctrl->box_band[j];
-  }
-  const int x = box[0] + blockIdx.x * 64 + threadIdx.x;
-  const int y = box[2] + blockIdx.y * 4 + threadIdx.y;
-  const int z = box[4] + blockIdx.z;
-  if (x > box[1] || y > box[3] || z > box[5]) return;
-  if (!interior(g, x, y, z)) return;
-  const int64_t i = cell_index(g, x, y, z);
-  if (tag[i] != TAG_FLUID) return;
-  const double p = phi[i];
-  float gx1, gy1, gz1, lap1;
-  phi_grad(g, phi, i, p, gx1, gy1, gz1, lap1);
-  const double uxe = ux[i], uye = uy[i], uze = uz[i];
-  float hin[Q7], hc[Q7];
-#pragma unroll
-  for (int k = 0; k < Q7; k++) hin[k] = hi[k * g.S + i];
-  collide_phase_cell(d, p, uxe, uye, uze, (double)gx1, (double)gy1, (double)gz1, hin, hc);
-#pragma unroll
-  for (int k = 0; k < Q7; k++) ho[k * g.S + i] = hc[k];
-  float fin[Q27], fc[Q27];
-#pragma unroll
-  for (int k = 0; k < Q27; k++) fin[k] = fi[k * g.S + i];
-  const double bx = use_bf ? (double)bfx[i] : 0.0;
-  const double by = use_bf ? (double)bfy[i] : 0.0;
-  const double bz = use_bf ? (double)bfz[i] : 0.0;
-  collide_hydro_cell(d, (double)rho[i], p, (double)lap1, (double)gx1, (double)gy1, (double)gz1, bx,
-                     by, bz, uxe, uye, uze, fin, fc);
-#pragma unroll
-  for (int k = 0; k < Q27; k++) fo[k * g.S + i] = fc[k];
-}
+// ---------------------------------------------------------------- body
 
-// Body update after stream(t), one 1024-thread workgroup:
+// Body update after stream(t) of step q, one 512-thread workgroup per scene:
 //   F/T += link momentum of the tiles intersecting this step's band box, in
 //          ascending tile order (contiguous chunks per thread, then waves in
 //          order);
 //   F/T += dead-cell impulses rho_phase(phi) u at the cells that turned solid at
 //          this step's retag (src/solid.js:87-101), in the same chunked order;
+//          a dead cell keeps the phi and u it had when it turned solid, which
+//          phi(t) holds;
 //   RigidSphere.step(1, g) (src/solid.js:104-210).
 // Then rolls the band boxes (the box of the old position serves the band
 // kernels of the next step, the box of the new position the next fused
 // launch) and, when check_band is set, verifies the centre moved at most
 // kBandDelta per axis.
-__global__ __launch_bounds__(1024) void k_body(Geom g, TileMap tm, Ctrl *ctrl,
-                                               const double *__restrict__ partials,
-                                               const int32_t *__restrict__ dead, int dead_cap,
-                                               const float *__restrict__ phi,
-                                               const float *__restrict__ ux,
-                                               const float *__restrict__ uy,
-                                               const float *__restrict__ uz, double rhoG,
-                                               double dRho, double gravity, int check_band,
-                                               int reset_lists) {
-  __shared__ double red[16][12];
+constexpr int kBodyThreads = 512;
+__global__ __launch_bounds__(kBodyThreads) void km_body(const Scene *__restrict__ scenes, int q,
+                                                int check_band, int reset_lists) {
+  __shared__ double red[kBodyThreads / 64][12];
+  const Scene &sc = scenes[blockIdx.x];
+  const Geom &g = sc.g;
+  const TileMap &tm = sc.tm;
+  Ctrl *ctrl = sc.ctrl;
   const int tid = threadIdx.x;
-  const lbm_sphere b0 = ctrl->sphere;
+  if (!sc.has_solid) {
+    if (tid == 0 && reset_lists) {
+      ctrl->n_fresh = 0;
+      ctrl->n_dead = 0;
+      ctrl->n_solid = 0;
+    }
+    return;
+  }
+  const double c0x = ctrl->sphere.cx, c0y = ctrl->sphere.cy, c0z = ctrl->sphere.cz;
+  const float *phi = sc.phi[sc.ep ^ (q & 1)];
   int box[6];
 #pragma unroll
   for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
@@ -530,32 +684,32 @@ __global__ __launch_bounds__(1024) void k_body(Geom g, TileMap tm, Ctrl *ctrl,
   const int ntiles = nbx * nby * nbz;
   double acc[12];
 #pragma unroll
-  for (int q = 0; q < 12; q++) acc[q] = 0;
+  for (int k = 0; k < 12; k++) acc[k] = 0;
   {
-    const int C = (ntiles + 1023) / 1024;
+    const int C = (ntiles + kBodyThreads - 1) / kBodyThreads;
     const int lo = tid * C, hi = min(ntiles, lo + C);
     for (int c = lo; c < hi; c++) {
       const int tx_ = c % nbx, ty_ = (c / nbx) % nby, tz_ = c / (nbx * nby);
       const int64_t t = (int64_t)(bx0 + tx_) + (int64_t)tm.tx * ((by0 + ty_) + (int64_t)tm.ty * (bz0 + tz_));
 #pragma unroll
-      for (int q = 0; q < 6; q++) acc[q] += partials[t * 6 + q];
+      for (int k = 0; k < 6; k++) acc[k] += sc.partials[t * 6 + k];
     }
   }
   int nd = ctrl->n_dead;
-  if (nd > dead_cap) nd = dead_cap;
+  if (nd > sc.list_cap) nd = sc.list_cap;
   {
-    const int C = (nd + 1023) / 1024;
+    const int C = (nd + kBodyThreads - 1) / kBodyThreads;
     const int lo = tid * C, hi = min(nd, lo + C);
     for (int c = lo; c < hi; c++) {
-      const int32_t i = dead[c];
+      const int32_t i = sc.dead[c];
       const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / g.nxny);
-      const double rhoPhase = rhoG + (double)phi[i] * dRho;
-      const double px = rhoPhase * (double)ux[i];
-      const double py = rhoPhase * (double)uy[i];
-      const double pz = rhoPhase * (double)uz[i];
-      const double rx = x + 0.5 - b0.cx;
-      const double ry = y + 0.5 - b0.cy;
-      const double rz = z + 0.5 - b0.cz;
+      const double rhoPhase = sc.d.rhoG + (double)phi[i] * sc.d.dRho;
+      const double px = rhoPhase * (double)sc.ux[i];
+      const double py = rhoPhase * (double)sc.uy[i];
+      const double pz = rhoPhase * (double)sc.uz[i];
+      const double rx = x + 0.5 - c0x;
+      const double ry = y + 0.5 - c0y;
+      const double rz = z + 0.5 - c0z;
       acc[6] += px;
       acc[7] += py;
       acc[8] += pz;
@@ -566,35 +720,35 @@ __global__ __launch_bounds__(1024) void k_body(Geom g, TileMap tm, Ctrl *ctrl,
   }
   const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
-  for (int q = 0; q < 12; q++) {
-    const double v = wave_sum(acc[q]);
-    if (lane == 0) red[wave][q] = v;
+  for (int k = 0; k < 12; k++) {
+    const double v = wave_sum(acc[k]);
+    if (lane == 0) red[wave][k] = v;
   }
   __syncthreads();
   if (tid != 0) return;
   double tot[12];
-  for (int q = 0; q < 12; q++) {
+  for (int k = 0; k < 12; k++) {
     double v = 0;
-    for (int w = 0; w < 16; w++) v += red[w][q];
-    tot[q] = v;
+    for (int w = 0; w < kBodyThreads / 64; w++) v += red[w][k];
+    tot[k] = v;
   }
-  lbm_sphere b = b0;
+  lbm_sphere b = ctrl->sphere;
   b.Fx += tot[0];
   b.Fy += tot[1];
   b.Fz += tot[2];
   b.Tx += tot[3];
   b.Ty += tot[4];
   b.Tz += tot[5];
-  for (int q = 0; q < 6; q++) ctrl->F_links[q] = tot[q];
+  for (int k = 0; k < 6; k++) ctrl->F_links[k] = tot[k];
   b.Fx += tot[6];
   b.Fy += tot[7];
   b.Fz += tot[8];
   b.Tx += tot[9];
   b.Ty += tot[10];
   b.Tz += tot[11];
-  sphere_step(b, 1.0, gravity);
+  sphere_step(b, 1.0, sc.d.gravity);
   if (check_band) {
-    const double d0 = b.cx - b0.cx, d1 = b.cy - b0.cy, d2 = b.cz - b0.cz;
+    const double d0 = b.cx - c0x, d1 = b.cy - c0y, d2 = b.cz - c0z;
     if (!(d0 <= kBandDelta && d0 >= -kBandDelta && d1 <= kBandDelta && d1 >= -kBandDelta &&
           d2 <= kBandDelta && d2 >= -kBandDelta))
       ctrl->err |= kErrBandEscape;

@@ -80,30 +80,19 @@ __device__ __forceinline__ void list_push(int *counter, int32_t *list, int cap, 
   else atomicOr(err, kErrListOverflow);
 }
 
-// retagCells (src/sim.js:214-244) over a box of cells [x0,x1]x[y0,y1]x[z0,z1].
-// mode 0: no solid; 1: the device sphere; 2: host mask + surface velocities.
-// copy_prev: also snapshot phi into phiPrev (the full-grid pass does; the fused
-// step's band retag does not, because there phiPrev is the other phi buffer).
+// retagCells (src/sim.js:214-244) for one cell.
+// mode 0: no solid; 1: the sphere b; 2: host mask + surface velocities.
+// copy_prev: also snapshot phi into phiPrev (the per-pass retag does; the fused
+// step does not, because there phiPrev is the other phi buffer).
 // Cells that turn fluid (fresh) or solid (dead) are appended to the lists, and
 // solid cells are counted.
-__global__ __launch_bounds__(256) void k_retag(Geom g, int mode, const lbm_sphere *__restrict__ sph,
-                                               const uint8_t *__restrict__ mask,
-                                               const float *__restrict__ mus, uint8_t *tag,
-                                               uint8_t *tagPrev, const float *__restrict__ phi,
-                                               float *__restrict__ phiPrev, int copy_prev,
-                                               float *usx, float *usy, float *usz, Ctrl *ctrl,
-                                               int32_t *fresh, int32_t *dead, int cap, int bx0,
-                                               int bx1, int by0, int by1, int bz0, int bz1,
-                                               int use_ctrl_box) {
-  int box[6] = {bx0, bx1, by0, by1, bz0, bz1};
-  if (use_ctrl_box) {
-#pragma unroll
-    for (int j = 0; j < 6; j++) box[j] = ctrl->box_band[j];
-  }
-  const int x = box[0] + blockIdx.x * 64 + threadIdx.x;
-  const int y = box[2] + blockIdx.y * 4 + threadIdx.y;
-  const int z = box[4] + blockIdx.z;
-  if (x > box[1] || y > box[3] || z > box[5]) return;
+__device__ __forceinline__ void retag_cell(const Geom &g, int mode, const lbm_sphere &b,
+                                           const uint8_t *__restrict__ mask,
+                                           const float *__restrict__ mus, uint8_t *tag,
+                                           uint8_t *tagPrev, const float *__restrict__ phi,
+                                           float *__restrict__ phiPrev, int copy_prev, float *usx,
+                                           float *usy, float *usz, Ctrl *ctrl, int32_t *fresh,
+                                           int32_t *dead, int cap, int x, int y, int z) {
   const int64_t i = cell_index(g, x, y, z);
   const uint8_t tp = tag[i];
   tagPrev[i] = tp;
@@ -115,7 +104,6 @@ __global__ __launch_bounds__(256) void k_retag(Geom g, int mode, const lbm_spher
   } else {
     bool solid = false;
     if (mode == 1) {
-      const lbm_sphere b = *sph;
       solid = sphere_contains(b, x, y, z);
       if (solid) {
         double tmp[3];
@@ -141,6 +129,21 @@ __global__ __launch_bounds__(256) void k_retag(Geom g, int mode, const lbm_spher
   usx[i] = vx;
   usy[i] = vy;
   usz[i] = vz;
+}
+
+// retagCells over the whole grid (per-pass API).
+__global__ __launch_bounds__(256) void k_retag(Geom g, int mode, const lbm_sphere *__restrict__ sph,
+                                               const uint8_t *__restrict__ mask,
+                                               const float *__restrict__ mus, uint8_t *tag,
+                                               uint8_t *tagPrev, const float *__restrict__ phi,
+                                               float *__restrict__ phiPrev, int copy_prev,
+                                               float *usx, float *usy, float *usz, Ctrl *ctrl,
+                                               int32_t *fresh, int32_t *dead, int cap) {
+  LBM_CELL_XYZ
+  lbm_sphere b{};
+  if (mode == 1) b = *sph;
+  retag_cell(g, mode, b, mask, mus, tag, tagPrev, phi, phiPrev, copy_prev, usx, usy, usz, ctrl,
+             fresh, dead, cap, x, y, z);
 }
 
 // Sorts a[0..count) ascending within one workgroup (bitonic).  Lists up to

@@ -20,6 +20,13 @@
 
 using namespace lbm;
 
+// Per-launch HIP event timing (lbm_set_timing / lbm_kernel_time).
+struct Timer {
+  bool on = false;
+  std::map<std::string, std::pair<double, int64_t>> total;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+};
+
 struct lbm_sim {
   int nx, ny, nz;
   Geom g;
@@ -65,10 +72,10 @@ struct lbm_sim {
   bool need_full_retag = true;  // next coupled step must retag the whole grid
   bool sphere_dev_valid = false;
   std::vector<lbm_link> links;  // kept by lbm_stream_hydro_macro_links
+  // Fused-path scene table (one scene) in device memory.
+  void *scene_dev = nullptr;
   // Timing.
-  bool timing = false;
-  std::map<std::string, std::pair<double, int64_t>> ktime;
-  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  Timer timer;
   std::string err;
 };
 
@@ -177,36 +184,37 @@ static int check_launch(lbm_sim *s, const char *what) {
 
 // Timing brackets around a launch (only when timing is on).
 struct TimedLaunch {
-  lbm_sim *s;
+  Timer &t;
+  hipStream_t st;
   const char *name;
   hipEvent_t a = nullptr, b = nullptr;
-  TimedLaunch(lbm_sim *s_, const char *n) : s(s_), name(n) {
-    if (s->timing) {
+  TimedLaunch(Timer &t_, hipStream_t st_, const char *n) : t(t_), st(st_), name(n) {
+    if (t.on) {
       hipEventCreate(&a);
       hipEventCreate(&b);
-      hipEventRecord(a, s->stream);
+      hipEventRecord(a, st);
     }
   }
   ~TimedLaunch() {
-    if (s->timing) {
-      hipEventRecord(b, s->stream);
-      s->pending.push_back({name, {a, b}});
+    if (t.on) {
+      hipEventRecord(b, st);
+      t.pending.push_back({name, {a, b}});
     }
   }
 };
 
-static void collect_timing(lbm_sim *s) {
-  for (auto &p : s->pending) {
+static void collect_timing(Timer &t) {
+  for (auto &p : t.pending) {
     float ms = 0;
     hipEventSynchronize(p.second.second);
     hipEventElapsedTime(&ms, p.second.first, p.second.second);
-    auto &e = s->ktime[p.first];
+    auto &e = t.total[p.first];
     e.first += ms;
     e.second += 1;
     hipEventDestroy(p.second.first);
     hipEventDestroy(p.second.second);
   }
-  s->pending.clear();
+  t.pending.clear();
 }
 
 // ------------------------------------------------------------------ lifetime
@@ -307,12 +315,12 @@ extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
 extern "C" void lbm_destroy(lbm_sim *s) {
   if (!s) return;
   if (s->stream) hipStreamSynchronize(s->stream);
-  collect_timing(s);
+  collect_timing(s->timer);
   void *ptrs[] = {s->fbuf[0], s->fbuf[1], s->hbuf[0], s->hbuf[1], s->phibuf[0], s->phibuf[1],
                   s->rho, s->ux, s->uy, s->uz, s->tag, s->tagPrev, s->usx, s->usy, s->usz, s->gx,
                   s->gy, s->gz, s->lap, s->bfx, s->bfy, s->bfz, s->fresh, s->dead,
                   s->level_scratch, s->ctrl, s->link_key, s->link_m, s->imp, s->scratch,
-                  s->mask_dev, s->mus_dev, s->partials};
+                  s->mask_dev, s->mus_dev, s->partials, s->scene_dev};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (s->ctrl_host) hipHostFree(s->ctrl_host);
@@ -416,8 +424,7 @@ static int retag_full(lbm_sim *s, int mode) {
   HIP_TRY(s, hipMemsetAsync(s->ctrl, 0, 4 * sizeof(int), s->stream));  // list counters
   k_retag<<<cell_grid(s), kCellBlock, 0, s->stream>>>(
       s->g, mode, &s->ctrl->sphere, s->mask_dev, s->mus_dev, s->tag, s->tagPrev, PHI(s),
-      PHIPREV(s), 1, s->usx, s->usy, s->usz, s->ctrl, s->fresh, s->dead, s->list_cap, 0,
-      s->nx - 1, 0, s->ny - 1, 0, s->nz - 1, 0);
+      PHIPREV(s), 1, s->usx, s->usy, s->usz, s->ctrl, s->fresh, s->dead, s->list_cap);
   int rc = check_launch(s, "k_retag");
   if (rc) return rc;
   k_sort_lists<<<2, 1024, 0, s->stream>>>(s->fresh, s->dead, s->ctrl, s->list_cap);
@@ -858,10 +865,17 @@ extern "C" int lbm_set_timing(lbm_sim *s, int on) {
   CHECK_SIM(s);
   int rc = sync(s);
   if (rc) return rc;
-  collect_timing(s);
-  s->timing = on != 0;
-  s->ktime.clear();
+  collect_timing(s->timer);
+  s->timer.on = on != 0;
+  s->timer.total.clear();
   return LBM_OK;
+}
+
+static void timer_read(Timer &t, const char *name, double *total_ms, int64_t *launches) {
+  collect_timing(t);
+  auto it = t.total.find(name);
+  if (total_ms) *total_ms = it == t.total.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == t.total.end() ? 0 : it->second.second;
 }
 
 extern "C" int lbm_kernel_time(lbm_sim *s, const char *name, double *total_ms, int64_t *launches) {
@@ -869,10 +883,7 @@ extern "C" int lbm_kernel_time(lbm_sim *s, const char *name, double *total_ms, i
   if (!name) return LBM_ERR_ARG;
   int rc = sync(s);
   if (rc) return rc;
-  collect_timing(s);
-  auto it = s->ktime.find(name);
-  if (total_ms) *total_ms = it == s->ktime.end() ? 0.0 : it->second.first;
-  if (launches) *launches = it == s->ktime.end() ? 0 : it->second.second;
+  timer_read(s->timer, name, total_ms, launches);
   return LBM_OK;
 }
 

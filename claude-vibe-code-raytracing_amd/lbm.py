@@ -117,6 +117,16 @@ def lib():
                                       C.POINTER(C.c_int64)]),
         "lbm_version": (C.c_char_p, []),
         "lbm_probe_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
+        "lbm_batch_create": (C.c_int, [C.POINTER(P)]),
+        "lbm_batch_destroy": (None, [P]),
+        "lbm_batch_last_error": (C.c_char_p, [P]),
+        "lbm_batch_add": (C.c_int, [P, P, SP]),
+        "lbm_batch_remove": (C.c_int, [P, P]),
+        "lbm_batch_size": (C.c_int, [P]),
+        "lbm_batch_step": (C.c_int, [P, C.c_int]),
+        "lbm_batch_set_timing": (C.c_int, [P, C.c_int]),
+        "lbm_batch_kernel_time": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -328,4 +338,57 @@ class Sim:
         cnt = C.c_int64(0)
         self._ck(self.L.lbm_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(cnt)),
                  "lbm_kernel_time")
+        return ms.value, cnt.value
+
+
+class Batch:
+    """Many simulations stepped by the same launches (struct lbm_batch)."""
+
+    def __init__(self):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.lbm_batch_create(C.byref(h))
+        if rc != 0:
+            raise LbmError(f"lbm_batch_create = {rc}: {L.lbm_last_error(None).decode()}")
+        self.h = h
+        self.L = L
+        self.members = []  # keeps the Sim and Sphere objects alive
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lbm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc, what):
+        if rc < 0:
+            raise LbmError(f"{what} = {rc}: {self.L.lbm_batch_last_error(self.h).decode()}")
+        return rc
+
+    def add(self, sim, sphere=None):
+        idx = self._ck(self.L.lbm_batch_add(self.h, sim.h, C.byref(sphere) if sphere is not None
+                                            else None), "lbm_batch_add")
+        self.members.append((sim, sphere))
+        return idx
+
+    def remove(self, sim):
+        self._ck(self.L.lbm_batch_remove(self.h, sim.h), "lbm_batch_remove")
+        self.members = [m for m in self.members if m[0] is not sim]
+
+    def step(self, nsteps):
+        self._ck(self.L.lbm_batch_step(self.h, int(nsteps)), "lbm_batch_step")
+
+    def set_timing(self, on):
+        self._ck(self.L.lbm_batch_set_timing(self.h, 1 if on else 0), "lbm_batch_set_timing")
+
+    def kernel_time(self, name):
+        ms = C.c_double(0)
+        cnt = C.c_int64(0)
+        self._ck(self.L.lbm_batch_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(cnt)),
+                 "lbm_batch_kernel_time")
         return ms.value, cnt.value

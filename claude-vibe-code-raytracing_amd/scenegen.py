@@ -68,3 +68,27 @@ def config1(nx, ny, nz, seed=1):
     vx, vy, vz = random_velocity(rng, 0.02)
     sphere = dict(cx=nx * 0.5, cy=0.2 * ny, cz=nz * 0.5, r=24.0, density=3.0, vx=vx, vy=vy, vz=vz)
     return dict(PARAMS), yw, sphere
+
+
+def config2(count=64, seed=2):
+    """Batched scenes: axes U{32..96}, liquid fill U[0.3, 0.6] with a perturbed
+    interface, one sphere each (r U[3, 10], density U[1.5, 4]) at rest at a
+    random position above the interface inside its clamp box; parameters as in
+    config1.  Returns a list of (dims, params, yw[nz, nx], sphere dict)."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        nx, ny, nz = rng.randint(32, 96), rng.randint(32, 96), rng.randint(32, 96)
+        fill = rng.uniform(0.3, 0.6)
+        yw, _ = waterline_modes(rng, nx, nz, (1.0 - fill) * ny)
+        r = rng.uniform(3.0, 10.0)
+        density = rng.uniform(1.5, 4.0)
+        m = r + 1.5
+        top = float(yw.min()) - r - 1.0  # sphere fully above the interface (y is down)
+        lo_y, hi_y = m, min(top, ny - 1 - m)
+        cy = rng.uniform(lo_y, hi_y) if hi_y > lo_y else lo_y
+        cx = rng.uniform(m, nx - 1 - m)
+        cz = rng.uniform(m, nz - 1 - m)
+        out.append(((nx, ny, nz), dict(PARAMS), yw,
+                    dict(cx=cx, cy=cy, cz=cz, r=r, density=density, vx=0.0, vy=0.0, vz=0.0)))
+    return out

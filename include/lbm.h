@@ -180,6 +180,28 @@ int lbm_absorb_dead_cells(lbm_sim *sim, lbm_sphere *solid);
  * per-pass path in the last bits.  solid may be NULL (no body). */
 int lbm_step(lbm_sim *sim, lbm_sphere *solid, int nsteps);
 
+/* ------------------------------------------------------------------ batches */
+/* Many simulations stepped together.  No reference counterpart: the reference
+ * runs one Sim per Web Worker (src/worker.js:11-13); a batch steps any number
+ * of independent Sims (each with its own size, parameters and sphere) with
+ * launches that cover all of them, so small scenes fill the GPU together.
+ * Every member's results are those of lbm_step on it alone, bit for bit. */
+typedef struct lbm_batch lbm_batch;
+int lbm_batch_create(lbm_batch **out);
+void lbm_batch_destroy(lbm_batch *batch);
+/* Message for the last failed call on this batch. */
+const char *lbm_batch_last_error(const lbm_batch *batch);
+/* Adds sim (solid may be NULL); the batch keeps both pointers and updates
+ * *solid after every lbm_batch_step.  Returns the member index (>= 0). */
+int lbm_batch_add(lbm_batch *batch, lbm_sim *sim, lbm_sphere *solid);
+int lbm_batch_remove(lbm_batch *batch, lbm_sim *sim);
+int lbm_batch_size(const lbm_batch *batch);
+/* nsteps coupled steps of every member (lbm_step semantics). */
+int lbm_batch_step(lbm_batch *batch, int nsteps);
+/* Per-kernel HIP event timing of lbm_batch_step, as lbm_set_timing. */
+int lbm_batch_set_timing(lbm_batch *batch, int on);
+int lbm_batch_kernel_time(lbm_batch *batch, const char *name, double *total_ms, int64_t *launches);
+
 /* ------------------------------------------------------------------ fields */
 
 /* Number of elements of a field (n, n*27 or n*7). */
