@@ -41,55 +41,60 @@ constexpr int kErrListOverflow = 1;
 constexpr int kErrBandEscape = 2;
 constexpr int kErrFreshCapacity = 4;
 
-// Tiles of TX x TY x 1 cells (TX * TY = 256, four waves of 64 consecutive x)
-// launched on a 1-D grid in XCD-aware order.
-//  * Tile width: the widest of 256, 128, 64 cells that divides the row padded
-//    to 64.  Each population plane is then read and written in runs of TX
-//    consecutive cells per workgroup; with 34 planes in flight, 1 KB runs keep
-//    HBM pages open far better than 256 B runs (measured on MI355X: 4.4 vs
-//    3.9 TB/s for the 34-plane pull pattern).
+// Tiles of 256 consecutive cells of one z-plane (four waves of 64), launched
+// on a 1-D grid in XCD-aware order.
+//  * Shape: a tile is a run of the x-fastest cell order inside a plane, so it
+//    covers one row of a 256-wide grid, several rows of a narrow one, and
+//    leaves no lane idle except at the end of a plane.  Each population plane
+//    is read and written in 1 KB runs per workgroup; with 34 planes in flight,
+//    1 KB runs keep HBM pages open far better than 256 B runs (measured on
+//    MI355X: 4.4 vs 3.9 TB/s for the 34-plane pull pattern).
 //  * Order: the hardware hands consecutive workgroups to the eight XCDs in
 //    turn, and each XCD has its own L2; mapping physical workgroup p to tile
 //    (p % 8) * (T8 / 8) + p / 8 gives every XCD a contiguous run of tiles (a
 //    slab of z-planes), so tiles that share halo rows run on the same L2.
 struct TileMap {
-  int TX, TY;      // tile size in cells (TX * TY = 256)
-  int tx, ty, tz;  // tiles per axis
-  int T;           // tx * ty * tz
-  int T8;          // T rounded up to a multiple of 8 (the launched grid)
+  int nx, nxny;  // row length and plane size in cells
+  int tpp;       // tiles per z-plane: ceil(nx * ny / 256)
+  int T;         // tpp * nz
+  int T8;        // T rounded up to a multiple of 8 (the launched grid)
 };
 
 inline TileMap make_tile_map(int nx, int ny, int nz) {
   TileMap m;
-  const int row = (nx + 63) / 64 * 64;
-  m.TX = row % 256 == 0 ? 256 : row % 128 == 0 ? 128 : 64;
-  m.TY = 256 / m.TX;
-  m.tx = row / m.TX;
-  m.ty = (ny + m.TY - 1) / m.TY;
-  m.tz = nz;
-  m.T = m.tx * m.ty * m.tz;
+  m.nx = nx;
+  m.nxny = nx * ny;
+  m.tpp = (m.nxny + 255) / 256;
+  m.T = m.tpp * nz;
   m.T8 = (m.T + 7) / 8 * 8;
   return m;
 }
 
-// Tile of this workgroup and the cell of this thread; false for the padding
-// workgroups past T.  Workgroups are 64 x 4 threads (threadIdx.y is the wave).
-__device__ __forceinline__ bool map_tile(const TileMap &m, int &tile, int &bx, int &by, int &bz) {
-  const int p = blockIdx.x;
-  tile = (p & 7) * (m.T8 >> 3) + (p >> 3);
+// Tile of launch-local workgroup lb (XCD-aware order) and its plane z and
+// first plane-local cell c0; false for the padding workgroups past T.
+__device__ __forceinline__ bool map_tile(const TileMap &m, int lb, int &tile, int &z, int &c0) {
+  tile = (lb & 7) * (m.T8 >> 3) + (lb >> 3);
   if (tile >= m.T) return false;
-  bx = tile % m.tx;
-  by = (tile / m.tx) % m.ty;
-  bz = tile / (m.tx * m.ty);
+  z = tile / m.tpp;
+  c0 = (tile - z * m.tpp) * 256;
   return true;
 }
 
-__device__ __forceinline__ void tile_cell(const TileMap &m, int bx, int by, int bz, int &x, int &y,
-                                          int &z) {
-  const int t = threadIdx.y * 64 + threadIdx.x;
-  x = bx * m.TX + t % m.TX;
-  y = by * m.TY + t / m.TX;
-  z = bz;
+// Cell of this thread (workgroups are 64 x 4 threads, threadIdx.y the wave);
+// false past the end of the plane.
+__device__ __forceinline__ bool tile_cell(const TileMap &m, int c0, int &x, int &y) {
+  const int c = c0 + threadIdx.y * 64 + threadIdx.x;
+  y = c / m.nx;
+  x = c - y * m.nx;
+  return c < m.nxny;
+}
+
+// Plane-local tile range [lo, hi] that holds the cells of rows box[2]..box[3]
+// between x = box[0] and x = box[1]: every cell of a box plane lies in one of
+// these tiles.
+__device__ __forceinline__ void box_tiles(const TileMap &m, const int box[6], int &lo, int &hi) {
+  lo = (box[2] * m.nx + box[0]) >> 8;
+  hi = (box[3] * m.nx + box[1]) >> 8;
 }
 
 __device__ __forceinline__ int64_t cell_index(const Geom &g, int x, int y, int z) {

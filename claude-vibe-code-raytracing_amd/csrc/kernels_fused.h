@@ -295,10 +295,11 @@ __device__ __forceinline__ void tile_momentum(double mom[6], double *partials, i
 
 // ---------------------------------------------------------------- scenes
 
-// One simulation of a scene set.  Buffer pairs: at a steady step of global
-// parity q the post-collision populations are f[ef ^ q] (written by the
-// previous step) and the step writes f[ef ^ q ^ 1]; phi(t) is phi[ep ^ q] and
-// phi(t+1) phi[ep ^ q ^ 1].
+// One simulation of a scene set.  Buffer pairs are stored by step parity: a
+// step of parity q reads the post-collision populations f[q] (written by the
+// previous step) and writes f[q ^ 1]; it reads phi(t) from phi[q] and writes
+// phi(t+1) to phi[q ^ 1].  The first collide of a call reads the reference's f
+// from f[1] and writes f[0].
 struct Scene {
   Geom g;
   TileMap tm;
@@ -312,14 +313,22 @@ struct Scene {
   int *level_scratch;
   Ctrl *ctrl;
   double *partials;
+  double rhoL;              // liquid density, as set (fresh cells use it)
   int list_cap;
-  int ef, ep;               // 0 or 1
-  double rhoL;              // liquid density, as set (fresh cells use rhoL - rhoG)
   int has_solid;
   int use_bf;               // external force armed for the first step of the call
   int cgx, cgy;             // full-grid blocks of 64x4 cells: cgx * cgy * nz
   int bgx, bgy, bgz;        // band blocks of 64x4 cells (covers the largest band box)
 };
+
+// Buffer of parity q (0 or 1) of a pair.  Both pointers are loaded and one is
+// selected, so the load does not wait on q.
+template <typename T>
+__device__ __forceinline__ T *par(T *const pair[2], int q) {
+  T *a = pair[0];
+  T *b = pair[1];
+  return q ? b : a;
+}
 
 // Scene of block b: off[s] <= b < off[s + 1].
 __device__ __forceinline__ int find_scene(const int *__restrict__ off, int n, int b) {
@@ -332,18 +341,39 @@ __device__ __forceinline__ int find_scene(const int *__restrict__ off, int n, in
   return lo;
 }
 
-// Tile of a block of a tile-mapped launch (TileMap order, offsets multiples of 8
-// so the XCD-aware order holds inside every scene).
-__device__ __forceinline__ bool scene_tile(const Scene &sc, int lb, int &tile, int &bx, int &by,
-                                           int &bz) {
-  const TileMap &m = sc.tm;
-  tile = (lb & 7) * (m.T8 >> 3) + (lb >> 3);
-  if (tile >= m.T) return false;
-  bx = tile % m.tx;
-  by = (tile / m.tx) % m.ty;
-  bz = tile / (m.tx * m.ty);
-  return true;
+
+// Cell of block lb of a 64x4-block launch over the whole grid or over the band
+// box; false when outside.
+template <bool BAND>
+__device__ __forceinline__ bool block_cell(const Scene &sc, int lb, int &x, int &y, int &z) {
+  if (BAND) {
+    const int *box = sc.ctrl->box_band;
+    x = box[0] + (lb % sc.bgx) * 64 + threadIdx.x;
+    y = box[2] + ((lb / sc.bgx) % sc.bgy) * 4 + threadIdx.y;
+    z = box[4] + lb / (sc.bgx * sc.bgy);
+    return x <= box[1] && y <= box[3] && z <= box[5];
+  }
+  x = (lb % sc.cgx) * 64 + threadIdx.x;
+  y = ((lb / sc.cgx) % sc.cgy) * 4 + threadIdx.y;
+  z = lb / (sc.cgx * sc.cgy);
+  return x < sc.g.nx && y < sc.g.ny;
 }
+
+// Every kernel comes in two launch forms: km_* over a scene table (a block finds
+// its scene by binary search in the prefix table off of block offsets) and k1_*
+// for one scene passed by value, whose descriptor arrives with the kernel
+// arguments and costs no dependent load at block start.
+#define LBM_SCENE_KERNELS(NAME, BODY, TPARAMS, TARGS, ...)                               \
+  TPARAMS __global__ __launch_bounds__(256) void km_##NAME(                             \
+      const Scene *__restrict__ scenes, const int *__restrict__ off, int nscenes,      \
+      LBM_PARAMS_##NAME) {                                                               \
+    const int si = find_scene(off, nscenes, blockIdx.x);                                 \
+    BODY TARGS(scenes[si], blockIdx.x - off[si], __VA_ARGS__);                           \
+  }                                                                                      \
+  TPARAMS __global__ __launch_bounds__(256) void k1_##NAME(const Scene sc,               \
+                                                           LBM_PARAMS_##NAME) {         \
+    BODY TARGS(sc, blockIdx.x, __VA_ARGS__);                                             \
+  }
 
 // ---------------------------------------------------------------- retag
 
@@ -351,86 +381,60 @@ __device__ __forceinline__ bool scene_tile(const Scene &sc, int lb, int &tile, i
 // (later steps).  No phi snapshot: in the fused form phiPrev is the other phi
 // buffer.
 template <bool BAND>
-__global__ __launch_bounds__(256) void km_retag(const Scene *__restrict__ scenes,
-                                                const int *__restrict__ off, int nscenes) {
-  const int si = find_scene(off, nscenes, blockIdx.x);
-  const Scene &sc = scenes[si];
-  const int lb = blockIdx.x - off[si];
+__device__ __forceinline__ void retag_body(const Scene &sc, int lb, int) {
   int x, y, z;
-  if (BAND) {
-    const int *box = sc.ctrl->box_band;
-    x = box[0] + (lb % sc.bgx) * 64 + threadIdx.x;
-    y = box[2] + ((lb / sc.bgx) % sc.bgy) * 4 + threadIdx.y;
-    z = box[4] + lb / (sc.bgx * sc.bgy);
-    if (x > box[1] || y > box[3] || z > box[5]) return;
-  } else {
-    x = (lb % sc.cgx) * 64 + threadIdx.x;
-    y = ((lb / sc.cgx) % sc.cgy) * 4 + threadIdx.y;
-    z = lb / (sc.cgx * sc.cgy);
-    if (x >= sc.g.nx || y >= sc.g.ny) return;
-  }
+  if (!block_cell<BAND>(sc, lb, x, y, z)) return;
   lbm_sphere b{};
   if (sc.has_solid) b = sc.ctrl->sphere;
   retag_cell(sc.g, sc.has_solid ? 1 : 0, b, nullptr, nullptr, sc.tag, sc.tagPrev, nullptr,
              nullptr, 0, sc.usx, sc.usy, sc.usz, sc.ctrl, sc.fresh, sc.dead, sc.list_cap, x, y, z);
 }
+#define LBM_PARAMS_retag int unused
+LBM_SCENE_KERNELS(retag, retag_body, template <bool BAND>, <BAND>, unused)
 
 // Sorts both lists of the scene's last retag, then runs handleFreshCells; one
-// workgroup per scene.  Upcoming step p: phi is phi[ep ^ p]; the post-stream
-// populations of fresh cells go to the reference's f (f[ef ^ 1]) before the
-// first step of a call, and in place into f[ef ^ p] for band updates.
-__global__ __launch_bounds__(1024) void km_sort_fresh(const Scene *__restrict__ scenes, int p,
-                                                      int prologue) {
+// workgroup per scene.  Upcoming step p: phi is phi[p]; the post-stream
+// populations of fresh cells go to the reference's f (f[1]) before the first
+// step of a call, and in place into f[p] for band updates.
+__device__ __forceinline__ void sort_fresh_body(const Scene &sc, int p, int prologue) {
   __shared__ int32_t sh[kSortLds];
-  const Scene &sc = scenes[blockIdx.x];
   int nf = sc.ctrl->n_fresh, nd = sc.ctrl->n_dead;
   if (nf > sc.list_cap) nf = sc.list_cap;
   if (nd > sc.list_cap) nd = sc.list_cap;
   sort_list_block(sc.fresh, nf, sh);
   sort_list_block(sc.dead, nd, sh);
-  p &= 1;  // buffer parity of the upcoming step
-  const int fi = prologue ? (sc.ef ^ 1) : (sc.ef ^ p);
-  float *phi = sc.phi[sc.ep ^ p];
+  const int fi = prologue ? 1 : (p & 1);
+  float *phi = par(sc.phi, p & 1);
   fresh_block(sc.g, sc.d.rhoG, sc.rhoL, sc.fresh, nf, sc.level_scratch, sc.tag, phi, phi, phi,
-              sc.rho, sc.ux, sc.uy, sc.uz, sc.f[fi], sc.h[fi]);
+              sc.rho, sc.ux, sc.uy, sc.uz, par(sc.f, fi), par(sc.h, fi));
+}
+__global__ __launch_bounds__(1024) void km_sort_fresh(const Scene *__restrict__ scenes, int p,
+                                                      int prologue) {
+  sort_fresh_body(scenes[blockIdx.x], p, prologue);
+}
+__global__ __launch_bounds__(1024) void k1_sort_fresh(const Scene sc, int p, int prologue) {
+  sort_fresh_body(sc, p, prologue);
 }
 
 // ---------------------------------------------------------------- collide
 
 // Gradients + collide for the fluid cells of the whole grid (first step of a
-// call: post-stream populations in f[ef ^ 1] -> post-collision f[ef], the
-// external force if armed) or of the band box after step p - 1 (in place in
-// f[ef ^ p]).
+// call: post-stream populations in f[1] -> post-collision f[0], the external
+// force if armed) or of the band box before step p (in place in f[p]).
 template <bool BAND>
-__global__ __launch_bounds__(256) void km_collide(const Scene *__restrict__ scenes,
-                                                  const int *__restrict__ off, int nscenes,
-                                                  int p) {
-  const int si = find_scene(off, nscenes, blockIdx.x);
-  const Scene &sc = scenes[si];
+__device__ __forceinline__ void collide_body(const Scene &sc, int lb, int p) {
   const Geom &g = sc.g;
-  const int lb = blockIdx.x - off[si];
   int x, y, z;
-  if (BAND) {
-    const int *box = sc.ctrl->box_band;
-    x = box[0] + (lb % sc.bgx) * 64 + threadIdx.x;
-    y = box[2] + ((lb / sc.bgx) % sc.bgy) * 4 + threadIdx.y;
-    z = box[4] + lb / (sc.bgx * sc.bgy);
-    if (x > box[1] || y > box[3] || z > box[5]) return;
-  } else {
-    x = (lb % sc.cgx) * 64 + threadIdx.x;
-    y = ((lb / sc.cgx) % sc.cgy) * 4 + threadIdx.y;
-    z = lb / (sc.cgx * sc.cgy);
-    if (x >= g.nx || y >= g.ny) return;
-  }
+  if (!block_cell<BAND>(sc, lb, x, y, z)) return;
   if (!interior(g, x, y, z)) return;
   const int64_t i = cell_index(g, x, y, z);
   if (sc.tag[i] != TAG_FLUID) return;
-  p &= 1;  // buffer parity of the upcoming step
-  const float *fi = BAND ? sc.f[sc.ef ^ p] : sc.f[sc.ef ^ 1];
-  float *fo = BAND ? sc.f[sc.ef ^ p] : sc.f[sc.ef];
-  const float *hi = BAND ? sc.h[sc.ef ^ p] : sc.h[sc.ef ^ 1];
-  float *ho = BAND ? sc.h[sc.ef ^ p] : sc.h[sc.ef];
-  const float *phi = sc.phi[sc.ep ^ p];
+  p &= 1;
+  const float *fi = BAND ? par(sc.f, p) : sc.f[1];
+  float *fo = BAND ? par(sc.f, p) : sc.f[0];
+  const float *hi = BAND ? par(sc.h, p) : sc.h[1];
+  float *ho = BAND ? par(sc.h, p) : sc.h[0];
+  const float *phi = BAND ? par(sc.phi, p) : sc.phi[0];
   const double pc = phi[i];
   float gx1, gy1, gz1, lap1;
   phi_grad(g, phi, i, pc, gx1, gy1, gz1, lap1);
@@ -453,6 +457,8 @@ __global__ __launch_bounds__(256) void km_collide(const Scene *__restrict__ scen
 #pragma unroll
   for (int k = 0; k < Q27; k++) fo[k * g.S + i] = fc[k];
 }
+#define LBM_PARAMS_collide int p
+LBM_SCENE_KERNELS(collide, collide_body, template <bool BAND>, <BAND>, p)
 
 // ---------------------------------------------------------------- phase stream
 
@@ -460,20 +466,15 @@ __global__ __launch_bounds__(256) void km_collide(const Scene *__restrict__ scen
 // (src/sim.js:376-397, 549-552); other cells keep phi (the reference never
 // updates them).  Source tags are read first and every load address is chosen
 // without branching, so the seven loads are in flight together.
-__global__ __launch_bounds__(256) void km_phase_stream(const Scene *__restrict__ scenes,
-                                                       const int *__restrict__ off, int nscenes,
-                                                       int q) {
-  const int si = find_scene(off, nscenes, blockIdx.x);
-  const Scene &sc = scenes[si];
+__device__ __forceinline__ void phase_stream_body(const Scene &sc, int lb, int q) {
   const Geom &g = sc.g;
-  int tile, bx, by, bz, x, y, z;
-  if (!scene_tile(sc, blockIdx.x - off[si], tile, bx, by, bz)) return;
-  tile_cell(sc.tm, bx, by, bz, x, y, z);
-  if (x >= g.nx || y >= g.ny) return;
+  int tile, z, c0, x, y;
+  if (!map_tile(sc.tm, lb, tile, z, c0)) return;
+  if (!tile_cell(sc.tm, c0, x, y)) return;
   q &= 1;
-  const float *h2 = sc.h[sc.ef ^ q];
-  const float *phi = sc.phi[sc.ep ^ q];
-  float *phi_next = sc.phi[sc.ep ^ q ^ 1];
+  const float *h2 = par(sc.h, q);
+  const float *phi = par(sc.phi, q);
+  float *phi_next = par(sc.phi, q ^ 1);
   const uint8_t *tag = sc.tag;
   const int64_t i = cell_index(g, x, y, z);
   if (tag[i] != TAG_FLUID || !interior(g, x, y, z)) {
@@ -499,6 +500,8 @@ __global__ __launch_bounds__(256) void km_phase_stream(const Scene *__restrict__
   const double pp = phiSum < 0 ? 0 : phiSum > 1 ? 1 : phiSum;
   phi_next[i] = (float)pp;
 }
+#define LBM_PARAMS_phase_stream int q
+LBM_SCENE_KERNELS(phase_stream, phase_stream_body, , , q)
 
 // ---------------------------------------------------------------- stream + collide
 
@@ -509,23 +512,19 @@ __global__ __launch_bounds__(256) void km_phase_stream(const Scene *__restrict__
 // written here.  first: the first step of the call (the armed external force
 // applies to its macroscopic update).
 template <bool COLLIDE>
-__global__ __launch_bounds__(256) void km_stream_collide(const Scene *__restrict__ scenes,
-                                                         const int *__restrict__ off, int nscenes,
-                                                         int q, int first) {
-  const int si = find_scene(off, nscenes, blockIdx.x);
-  const Scene &sc = scenes[si];
+__device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int q, int first) {
   const Geom &g = sc.g;
   const Derived &d = sc.d;
-  int tile, bx, by, bz, x, y, z;
-  if (!scene_tile(sc, blockIdx.x - off[si], tile, bx, by, bz)) return;
-  tile_cell(sc.tm, bx, by, bz, x, y, z);
+  int tile, z, c0, x, y;
+  if (!map_tile(sc.tm, lb, tile, z, c0)) return;
+  const bool in_plane = tile_cell(sc.tm, c0, x, y);
   q &= 1;
-  const float *__restrict__ f2 = sc.f[sc.ef ^ q];
-  const float *__restrict__ h2 = sc.h[sc.ef ^ q];
-  float *__restrict__ fo = sc.f[sc.ef ^ q ^ 1];
-  float *__restrict__ ho = sc.h[sc.ef ^ q ^ 1];
-  const float *__restrict__ phi = sc.phi[sc.ep ^ q];
-  float *__restrict__ phi_next = sc.phi[sc.ep ^ q ^ 1];
+  const float *__restrict__ f2 = par(sc.f, q);
+  const float *__restrict__ h2 = par(sc.h, q);
+  float *__restrict__ fo = par(sc.f, q ^ 1);
+  float *__restrict__ ho = par(sc.h, q ^ 1);
+  const float *__restrict__ phi = par(sc.phi, q);
+  float *__restrict__ phi_next = par(sc.phi, q ^ 1);
   const uint8_t *__restrict__ tag = sc.tag;
   const Ctrl *__restrict__ ctrl = sc.ctrl;
   double mom[6] = {0, 0, 0, 0, 0, 0};
@@ -533,10 +532,13 @@ __global__ __launch_bounds__(256) void km_stream_collide(const Scene *__restrict
   int box[6];
 #pragma unroll
   for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
-  // Link momentum can only arise next to the sphere, which lies inside the band.
-  const bool tile_in_band = bx * sc.tm.TX <= box[1] && bx * sc.tm.TX + sc.tm.TX - 1 >= box[0] &&
-                            by * sc.tm.TY <= box[3] && by * sc.tm.TY + sc.tm.TY - 1 >= box[2] &&
-                            bz >= box[4] && bz <= box[5];
+  // Link momentum can only arise next to the sphere, which lies inside the band:
+  // the tiles of the band's planes between its first and last cell store their
+  // sums (zero or not), and the body update reads exactly those.
+  int tlo, thi;
+  box_tiles(sc.tm, box, tlo, thi);
+  const int lt = tile - z * sc.tm.tpp;
+  const bool tile_in_band = z >= box[4] && z <= box[5] && lt >= tlo && lt <= thi;
   const bool exchange = sc.has_solid != 0 && tile_in_band;
   const int use_bf = first && sc.use_bf;
   double cx = 0, cy = 0, cz = 0;
@@ -545,7 +547,7 @@ __global__ __launch_bounds__(256) void km_stream_collide(const Scene *__restrict
     cy = ctrl->sphere.cy;
     cz = ctrl->sphere.cz;
   }
-  if (x < g.nx && y < g.ny) {
+  if (in_plane) {
     const int64_t i = cell_index(g, x, y, z);
     const bool fluid = tag[i] == TAG_FLUID && interior(g, x, y, z);
     if (!fluid) {
@@ -640,13 +642,16 @@ __global__ __launch_bounds__(256) void km_stream_collide(const Scene *__restrict
   (void)linked;
   if (exchange) tile_momentum(mom, sc.partials, tile);
 }
+#define LBM_PARAMS_stream_collide int q, int first
+LBM_SCENE_KERNELS(stream_collide, stream_collide_body, template <bool COLLIDE>, <COLLIDE>, q,
+                  first)
 
 // ---------------------------------------------------------------- body
 
 // Body update after stream(t) of step q, one 512-thread workgroup per scene:
-//   F/T += link momentum of the tiles intersecting this step's band box, in
-//          ascending tile order (contiguous chunks per thread, then waves in
-//          order);
+//   F/T += link momentum of the band's tiles (in each plane of this step's
+//          band box, the tiles from its first to its last cell), in ascending
+//          tile order (contiguous chunks per thread, then waves in order);
 //   F/T += dead-cell impulses rho_phase(phi) u at the cells that turned solid at
 //          this step's retag (src/solid.js:87-101), in the same chunked order;
 //          a dead cell keeps the phi and u it had when it turned solid, which
@@ -657,10 +662,9 @@ __global__ __launch_bounds__(256) void km_stream_collide(const Scene *__restrict
 // launch) and, when check_band is set, verifies the centre moved at most
 // kBandDelta per axis.
 constexpr int kBodyThreads = 512;
-__global__ __launch_bounds__(kBodyThreads) void km_body(const Scene *__restrict__ scenes, int q,
-                                                int check_band, int reset_lists) {
+__device__ __forceinline__ void body_body(const Scene &sc, int q, int check_band,
+                                          int reset_lists) {
   __shared__ double red[kBodyThreads / 64][12];
-  const Scene &sc = scenes[blockIdx.x];
   const Geom &g = sc.g;
   const TileMap &tm = sc.tm;
   Ctrl *ctrl = sc.ctrl;
@@ -674,14 +678,15 @@ __global__ __launch_bounds__(kBodyThreads) void km_body(const Scene *__restrict_
     return;
   }
   const double c0x = ctrl->sphere.cx, c0y = ctrl->sphere.cy, c0z = ctrl->sphere.cz;
-  const float *phi = sc.phi[sc.ep ^ (q & 1)];
+  const float *phi = par(sc.phi, q & 1);
   int box[6];
 #pragma unroll
   for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
-  const int bx0 = box[0] / tm.TX, nbx = box[1] / tm.TX - bx0 + 1;
-  const int by0 = box[2] / tm.TY, nby = box[3] / tm.TY - by0 + 1;
+  int tlo, thi;
+  box_tiles(tm, box, tlo, thi);
+  const int nbt = thi - tlo + 1;
   const int bz0 = box[4], nbz = box[5] - box[4] + 1;
-  const int ntiles = nbx * nby * nbz;
+  const int ntiles = nbt * nbz;
   double acc[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) acc[k] = 0;
@@ -689,8 +694,8 @@ __global__ __launch_bounds__(kBodyThreads) void km_body(const Scene *__restrict_
     const int C = (ntiles + kBodyThreads - 1) / kBodyThreads;
     const int lo = tid * C, hi = min(ntiles, lo + C);
     for (int c = lo; c < hi; c++) {
-      const int tx_ = c % nbx, ty_ = (c / nbx) % nby, tz_ = c / (nbx * nby);
-      const int64_t t = (int64_t)(bx0 + tx_) + (int64_t)tm.tx * ((by0 + ty_) + (int64_t)tm.ty * (bz0 + tz_));
+      const int tz_ = c / nbt;
+      const int64_t t = (int64_t)(bz0 + tz_) * tm.tpp + tlo + (c - tz_ * nbt);
 #pragma unroll
       for (int k = 0; k < 6; k++) acc[k] += sc.partials[t * 6 + k];
     }
@@ -763,6 +768,14 @@ __global__ __launch_bounds__(kBodyThreads) void km_body(const Scene *__restrict_
     ctrl->n_dead = 0;
     ctrl->n_solid = 0;
   }
+}
+__global__ __launch_bounds__(kBodyThreads) void km_body(const Scene *__restrict__ scenes, int q,
+                                                        int check_band, int reset_lists) {
+  body_body(scenes[blockIdx.x], q, check_band, reset_lists);
+}
+__global__ __launch_bounds__(kBodyThreads) void k1_body(const Scene sc, int q, int check_band,
+                                                        int reset_lists) {
+  body_body(sc, q, check_band, reset_lists);
 }
 
 }  // namespace lbm

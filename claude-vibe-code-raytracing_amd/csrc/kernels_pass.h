@@ -123,7 +123,6 @@ __device__ __forceinline__ void retag_cell(const Geom &g, int mode, const lbm_sp
     t = solid ? TAG_SOLID : TAG_FLUID;
     if (t == TAG_FLUID && tp == TAG_SOLID) list_push(&ctrl->n_fresh, fresh, cap, (int32_t)i, &ctrl->err);
     if (t == TAG_SOLID && tp == TAG_FLUID) list_push(&ctrl->n_dead, dead, cap, (int32_t)i, &ctrl->err);
-    if (t == TAG_SOLID) atomicAdd(&ctrl->n_solid, 1);
   }
   tag[i] = t;
   usx[i] = vx;

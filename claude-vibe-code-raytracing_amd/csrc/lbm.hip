@@ -72,8 +72,6 @@ struct lbm_sim {
   bool need_full_retag = true;  // next coupled step must retag the whole grid
   bool sphere_dev_valid = false;
   std::vector<lbm_link> links;  // kept by lbm_stream_hydro_macro_links
-  // Fused-path scene table (one scene) in device memory.
-  void *scene_dev = nullptr;
   // Timing.
   Timer timer;
   std::string err;
@@ -320,7 +318,7 @@ extern "C" void lbm_destroy(lbm_sim *s) {
                   s->rho, s->ux, s->uy, s->uz, s->tag, s->tagPrev, s->usx, s->usy, s->usz, s->gx,
                   s->gy, s->gz, s->lap, s->bfx, s->bfy, s->bfz, s->fresh, s->dead,
                   s->level_scratch, s->ctrl, s->link_key, s->link_m, s->imp, s->scratch,
-                  s->mask_dev, s->mus_dev, s->partials, s->scene_dev};
+                  s->mask_dev, s->mus_dev, s->partials};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (s->ctrl_host) hipHostFree(s->ctrl_host);
