@@ -35,8 +35,13 @@ import lbm  # noqa: E402
 import scenegen  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_LU = {"stream_collide": 281, "phase_stream": 33, "stream_macro": 297}
-STEP_BYTES_PER_LU = 297
+# Algorithmic bytes per lattice update of the dominant kernel and of the whole
+# step, per population storage: float32 (4 B per population) or a 16-bit
+# deviation (2 B).  Kernel: 34 populations read and written + tag + phi(t) +
+# phi(t+1); step: 34 populations read and written + phi read and written +
+# rho/u written + tag.
+KERNEL_BYTES_PER_LU = {"f32": 281, "f16": 145, "bf16": 145}
+STEP_BYTES_PER_LU = {"f32": 297, "f16": 161, "bf16": 161}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "stream_collide_256cube_traffic.json")
 
 
@@ -49,6 +54,8 @@ def measured_traffic(n):
     except (OSError, ValueError):
         return None
     return t["traffic_bytes"] if t.get("cells") == n ** 3 else None
+STORAGE_NAME = {"f32": "float32", "f16": "fp16-deviation", "bf16": "bf16-deviation"}
+ARITH = {"f32": "f64", "f16": "f32", "bf16": "f32"}
 METRIC = "coupled MLUPS + %HBM peak at 256^3; served scenes MLUPS, p99 frame latency"
 
 
@@ -103,6 +110,7 @@ def bench_batch(args, copy_gbps):
         sim = lbm.Sim(nx, ny, nz)
         sim.set_params(**params)
         sim.init_waterline_map(yw)
+        sim.set_storage(args.storage)
         b = lbm.make_sphere(nx, ny, nz, sph["cx"], sph["cy"], sph["cz"], sph["r"], sph["density"])
         batch.add(sim, b)
         keep.append((sim, b))
@@ -120,24 +128,26 @@ def bench_batch(args, copy_gbps):
     mlups = cells * args.steps / dt / 1e6
     ms_sc, cnt_sc = kt["stream_collide"]
     avg_s = ms_sc / max(cnt_sc, 1) / 1e3
-    achieved = BYTES_PER_LU["stream_collide"] * cells / avg_s / 1e9 if cnt_sc else None
+    kb = KERNEL_BYTES_PER_LU[args.storage]
+    achieved = kb * cells / avg_s / 1e9 if cnt_sc else None
     finite = all(bool(np.isfinite(s.get("phi")).all()) for s, _ in keep[:4])
     out = {
         "metric": "batched scenes aggregate MLUPS (config 2)",
         "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": ARITH[args.storage], "data": "synthetic",
         "config": {"workload": f"config 2: {len(scenes)} seeded scenes, axes 32..96, one sphere "
-                               f"each, float32 populations, one batch", "scenes": len(scenes),
-                   "cells": cells, "storage": "f32", "arithmetic": "f64", "seed": args.seed + 1},
+                               f"each, {STORAGE_NAME[args.storage]} populations, one batch",
+                   "scenes": len(scenes), "cells": cells, "storage": args.storage,
+                   "arithmetic": ARITH[args.storage], "seed": args.seed + 1},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": None, "kernel": "km_stream_collide (all scenes per launch)",
-                     "bytes_per_lu": BYTES_PER_LU["stream_collide"],
-                     "avg_ms": round(avg_s * 1e3, 4)},
-        "whole_step_hbm_frac": round(STEP_BYTES_PER_LU * mlups * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
+                     "bytes_per_lu": kb, "avg_ms": round(avg_s * 1e3, 4)},
+        "whole_step_hbm_frac": round(STEP_BYTES_PER_LU[args.storage] * mlups * 1e6 /
+                                     (HBM_PEAK_GBS * 1e9), 4),
         "copy_probe_gbps": round(copy_gbps, 1),
         "kernel_ms": {k: {"total": round(v[0], 3), "launches": v[1]} for k, v in kt.items()},
         "finite": finite,
@@ -161,6 +171,9 @@ def main():
                     help="single: the 256^3 scene (headline); batch: 64 seeded scenes of "
                          "32..96 cells per axis stepped together")
     ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--storage", choices=["f32", "f16", "bf16"], default="f32",
+                    help="population storage: f32 (reference-exact, FP64 arithmetic) or a "
+                         "16-bit deviation (FP32 arithmetic)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the coupled step on a single MI355X "
@@ -174,6 +187,7 @@ def main():
         return bench_batch(args, copy_gbps)
     n = args.size
     sim, b, params, yw, sph = build_scene(n, args.seed)
+    sim.set_storage(args.storage)
     print(f"[bench] scene {n}^3 built; warmup {args.warmup}", file=sys.stderr, flush=True)
     sim.coupled(b, args.warmup)
     sim.synchronize()
@@ -190,7 +204,8 @@ def main():
     mlups = cells * args.steps / dt / 1e6
     ms_sc, cnt_sc = kt["stream_collide"]
     avg_s = ms_sc / max(cnt_sc, 1) / 1e3
-    achieved = BYTES_PER_LU["stream_collide"] * cells / avg_s / 1e9 if cnt_sc else None
+    kb = KERNEL_BYTES_PER_LU[args.storage]
+    achieved = kb * cells / avg_s / 1e9 if cnt_sc else None
     # sanity: the state stays finite
     phi = sim.get("phi")
     finite = bool((phi == phi).all())
@@ -205,19 +220,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": ARITH[args.storage],
         "data": "synthetic",
         "config": {"workload": f"config 1: {n}^3 single scene, coupled step (D3Q27 hydro + D3Q7 "
-                               f"phase + sphere), float32 populations", "nx": n, "ny": n, "nz": n,
-                   "storage": "f32", "arithmetic": "f64", "seed": args.seed,
+                               f"phase + sphere), {STORAGE_NAME[args.storage]} populations",
+                   "nx": n, "ny": n, "nz": n, "storage": args.storage,
+                   "arithmetic": ARITH[args.storage], "seed": args.seed,
                    "sphere_r": sph["r"], "params": params},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": measured_traffic(n), "kernel": "k_stream_collide",
-                     "bytes_per_lu": BYTES_PER_LU["stream_collide"],
+                     "traffic": measured_traffic(n) if args.storage == "f32" else None,
+                     "kernel": "k1_stream_collide", "bytes_per_lu": kb,
                      "avg_ms": round(avg_s * 1e3, 4)},
-        "whole_step_hbm_frac": round(STEP_BYTES_PER_LU * mlups * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
+        "whole_step_hbm_frac": round(STEP_BYTES_PER_LU[args.storage] * mlups * 1e6 /
+                                     (HBM_PEAK_GBS * 1e9), 4),
         "copy_probe_gbps": round(copy_gbps, 1),
         "kernel_ms": {k: {"total": round(v[0], 3), "launches": v[1]} for k, v in kt.items()},
         "finite": finite,

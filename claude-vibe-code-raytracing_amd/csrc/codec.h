@@ -1,0 +1,71 @@
+// Population storage codecs of the fused coupled step.
+//
+// ST_F32 stores populations as the reference does (float32) and computes in
+// FP64, so results match src/sim.js.  ST_F16 and ST_BF16 store each population
+// as a 16-bit deviation from a per-direction baseline and compute in FP32:
+//   f_k is stored as f_k - w_k rho0 (rho0 = 1: the pressure proxy sum f
+//     stays near 1 in both phases, so the deviations are small);
+//   h_k is stored as h_k - w7_k phi_c, with phi_c the phi of the cell at the
+//     time of the store (phi(t) for the post-collision h(t), phi(t+1) for the
+//     post-stream h); the reader knows that phi: it is the phi array of the
+//     same parity, at the source cell of the pull.
+// In the bulk of either phase the deviations are near zero, where a 16-bit
+// float has its finest absolute resolution.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbm {
+
+enum : int { ST_F32 = 0, ST_F16 = 1, ST_BF16 = 2 };
+
+// Round a float to bfloat16, nearest even; NaN stays a quiet NaN.
+__device__ __forceinline__ uint16_t f32_to_bf16(float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+template <int KIND>
+struct Codec;
+
+// T: stored element; R: arithmetic of the collide and macroscopic update.
+template <>
+struct Codec<ST_F32> {
+  using T = float;
+  using R = double;
+  static constexpr int bytes = 4;
+  __device__ __forceinline__ static float ld(const T *p, int64_t a, float) { return p[a]; }
+  __device__ __forceinline__ static void st(T *p, int64_t a, float v, float) { p[a] = v; }
+};
+
+template <>
+struct Codec<ST_F16> {
+  using T = _Float16;
+  using R = float;
+  static constexpr int bytes = 2;
+  __device__ __forceinline__ static float ld(const T *p, int64_t a, float base) {
+    return (float)p[a] + base;
+  }
+  __device__ __forceinline__ static void st(T *p, int64_t a, float v, float base) {
+    p[a] = (T)(v - base);
+  }
+};
+
+template <>
+struct Codec<ST_BF16> {
+  using T = uint16_t;
+  using R = float;
+  static constexpr int bytes = 2;
+  __device__ __forceinline__ static float ld(const T *p, int64_t a, float base) {
+    return bf16_to_f32(p[a]) + base;
+  }
+  __device__ __forceinline__ static void st(T *p, int64_t a, float v, float base) {
+    p[a] = f32_to_bf16(v - base);
+  }
+};
+
+}  // namespace lbm

@@ -41,60 +41,55 @@ constexpr int kErrListOverflow = 1;
 constexpr int kErrBandEscape = 2;
 constexpr int kErrFreshCapacity = 4;
 
-// Tiles of 256 consecutive cells of one z-plane (four waves of 64), launched
-// on a 1-D grid in XCD-aware order.
-//  * Shape: a tile is a run of the x-fastest cell order inside a plane, so it
-//    covers one row of a 256-wide grid, several rows of a narrow one, and
-//    leaves no lane idle except at the end of a plane.  Each population plane
-//    is read and written in 1 KB runs per workgroup; with 34 planes in flight,
-//    1 KB runs keep HBM pages open far better than 256 B runs (measured on
-//    MI355X: 4.4 vs 3.9 TB/s for the 34-plane pull pattern).
+// Tiles of 256 consecutive cells in the x-fastest order (four waves of 64),
+// launched on a 1-D grid in XCD-aware order.
+//  * Shape: a tile covers one row of a 256-wide grid, several rows of a narrow
+//    one, and leaves no lane idle except in the last tile.  Each population
+//    plane is read and written in 1 KB runs per workgroup; with 34 planes in
+//    flight, long runs keep HBM pages open far better than 256 B runs
+//    (measured on MI355X: 4.4 vs 3.9 TB/s for the 34-plane pull pattern).
 //  * Order: the hardware hands consecutive workgroups to the eight XCDs in
 //    turn, and each XCD has its own L2; mapping physical workgroup p to tile
 //    (p % 8) * (T8 / 8) + p / 8 gives every XCD a contiguous run of tiles (a
 //    slab of z-planes), so tiles that share halo rows run on the same L2.
 struct TileMap {
   int nx, nxny;  // row length and plane size in cells
-  int tpp;       // tiles per z-plane: ceil(nx * ny / 256)
-  int T;         // tpp * nz
-  int T8;        // T rounded up to a multiple of 8 (the launched grid)
+  int T, T8;     // tiles, and rounded up to a multiple of 8 (the grid)
 };
+
+constexpr int kTileShift = 8;  // 256 cells per tile: one cell per lane
 
 inline TileMap make_tile_map(int nx, int ny, int nz) {
   TileMap m;
   m.nx = nx;
   m.nxny = nx * ny;
-  m.tpp = (m.nxny + 255) / 256;
-  m.T = m.tpp * nz;
+  const int64_t n = (int64_t)m.nxny * nz;
+  m.T = (int)((n + (1 << kTileShift) - 1) >> kTileShift);
   m.T8 = (m.T + 7) / 8 * 8;
   return m;
 }
 
-// Tile of launch-local workgroup lb (XCD-aware order) and its plane z and
-// first plane-local cell c0; false for the padding workgroups past T.
-__device__ __forceinline__ bool map_tile(const TileMap &m, int lb, int &tile, int &z, int &c0) {
-  tile = (lb & 7) * (m.T8 >> 3) + (lb >> 3);
-  if (tile >= m.T) return false;
-  z = tile / m.tpp;
-  c0 = (tile - z * m.tpp) * 256;
-  return true;
+// Tile of launch-local workgroup lb of a grid of T8 workgroups (XCD-aware
+// order); false for the padding workgroups past T.
+__device__ __forceinline__ bool map_tile(int T, int T8, int lb, int &tile) {
+  tile = (lb & 7) * (T8 >> 3) + (lb >> 3);
+  return tile < T;
 }
 
-// Cell of this thread (workgroups are 64 x 4 threads, threadIdx.y the wave);
-// false past the end of the plane.
-__device__ __forceinline__ bool tile_cell(const TileMap &m, int c0, int &x, int &y) {
-  const int c = c0 + threadIdx.y * 64 + threadIdx.x;
-  y = c / m.nx;
-  x = c - y * m.nx;
-  return c < m.nxny;
+// Cell coordinates of index i.
+__device__ __forceinline__ void cell_xyz(int nx, int nxny, int64_t i, int &x, int &y, int &z) {
+  z = (int)(i / nxny);
+  const int r = (int)(i - (int64_t)z * nxny);
+  y = r / nx;
+  x = r - y * nx;
 }
 
-// Plane-local tile range [lo, hi] that holds the cells of rows box[2]..box[3]
-// between x = box[0] and x = box[1]: every cell of a box plane lies in one of
-// these tiles.
-__device__ __forceinline__ void box_tiles(const TileMap &m, const int box[6], int &lo, int &hi) {
-  lo = (box[2] * m.nx + box[0]) >> 8;
-  hi = (box[3] * m.nx + box[1]) >> 8;
+// Tiles from the one holding the band box's first cell to the one holding its
+// last cell: every cell of the box lies in one of them.
+__host__ __device__ inline void box_tile_range(const TileMap &m, const int box[6], int &lo,
+                                               int &hi) {
+  lo = (int)(((int64_t)box[4] * m.nxny + (int64_t)box[2] * m.nx + box[0]) >> kTileShift);
+  hi = (int)(((int64_t)box[5] * m.nxny + (int64_t)box[3] * m.nx + box[1]) >> kTileShift);
 }
 
 __device__ __forceinline__ int64_t cell_index(const Geom &g, int x, int y, int z) {

@@ -32,6 +32,7 @@
 // the reference's collide(t+1) reads.
 #pragma once
 
+#include "codec.h"
 #include "device_common.h"
 #include "kernels_pass.h"
 
@@ -67,16 +68,17 @@ __device__ __forceinline__ bool in_box(const int box[6], int x, int y, int z) {
 }
 
 // Gradient and Laplacian of a phi array at interior cell i (src/sim.js:186-195),
-// rounded to float as the reference's gx/gy/gz/lap arrays store them.
+// rounded to float as the reference's gx/gy/gz/lap arrays store them.  R is the
+// arithmetic type (double for the reference-exact path).
+template <typename R>
 __device__ __forceinline__ void phi_grad(const Geom &g, const float *__restrict__ phi, int64_t i,
-                                         double c, float &gxo, float &gyo, float &gzo,
-                                         float &lapo) {
-  const double px = phi[i + 1], mx = phi[i - 1];
-  const double py = phi[i + g.nx], my = phi[i - g.nx];
-  const double pz = phi[i + g.nxny], mz = phi[i - g.nxny];
-  gxo = (float)(0.5 * (px - mx));
-  gyo = (float)(0.5 * (py - my));
-  gzo = (float)(0.5 * (pz - mz));
+                                         R c, float &gxo, float &gyo, float &gzo, float &lapo) {
+  const R px = phi[i + 1], mx = phi[i - 1];
+  const R py = phi[i + g.nx], my = phi[i - g.nx];
+  const R pz = phi[i + g.nxny], mz = phi[i - g.nxny];
+  gxo = (float)(R(0.5) * (px - mx));
+  gyo = (float)(R(0.5) * (py - my));
+  gzo = (float)(R(0.5) * (pz - mz));
   lapo = (float)(px + mx + py + my + pz + mz - 6 * c);
 }
 
@@ -84,126 +86,131 @@ __device__ __forceinline__ void phi_grad(const Geom &g, const float *__restrict_
 // evaluates ekx*vx + eky*vy + ekz*vz; a zero component adds a signed zero, which
 // leaves every nonzero sum unchanged, so only the nonzero terms are summed, in
 // the same x, y, z order.
-template <int EXK, int EYK, int EZK>
-__device__ __forceinline__ double edot(double vx, double vy, double vz) {
-  double s = 0;
+template <int EXK, int EYK, int EZK, typename R>
+__device__ __forceinline__ R edot(R vx, R vy, R vz) {
+  R s = 0;
   bool first = true;
   if (EXK != 0) {
     s = EXK > 0 ? vx : -vx;
     first = false;
   }
   if (EYK != 0) {
-    const double t = EYK > 0 ? vy : -vy;
+    const R t = EYK > 0 ? vy : -vy;
     s = first ? t : (EYK > 0 ? s + vy : s - vy);
     first = false;
   }
   if (EZK != 0) {
-    const double t = EZK > 0 ? vz : -vz;
+    const R t = EZK > 0 ? vz : -vz;
     s = first ? t : (EZK > 0 ? s + vz : s - vz);
     first = false;
   }
-  return first ? 0.0 : s;
+  return first ? R(0) : s;
 }
 
 // collidePhase for one cell (src/sim.js:341-360).
-__device__ __forceinline__ void collide_phase_cell(const Derived &d, double p, double uxi,
-                                                   double uyi, double uzi, double ggx, double ggy,
-                                                   double ggz, const float hin[Q7],
+template <typename R>
+__device__ __forceinline__ void collide_phase_cell(const Derived &d, R p, R uxi, R uyi, R uzi,
+                                                   R ggx, R ggy, R ggz, const float hin[Q7],
                                                    float hout[Q7]) {
-  const double gmag = sqrt(ggx * ggx + ggy * ggy + ggz * ggz) + 1e-12;
-  const double sharpen = d.sharpC * p * (1 - p);
-  const double Sx = sharpen * d.sharpFac * ggx / gmag;
-  const double Sy = sharpen * d.sharpFac * ggy / gmag;
-  const double Sz = sharpen * d.sharpFac * ggz / gmag;
-  const double wp0 = (double)w7(0) * p;
-  const double wp1 = (double)w7(1) * p;
-  const double it = d.invTauPhi;
+  const R gmag = sqrt(ggx * ggx + ggy * ggy + ggz * ggz) + R(1e-12);
+  const R sharpen = R(d.sharpC) * p * (1 - p);
+  const R Sx = sharpen * R(d.sharpFac) * ggx / gmag;
+  const R Sy = sharpen * R(d.sharpFac) * ggy / gmag;
+  const R Sz = sharpen * R(d.sharpFac) * ggz / gmag;
+  const R wp0 = (R)w7(0) * p;
+  const R wp1 = (R)w7(1) * p;
+  const R it = R(d.invTauPhi);
+  const R c7 = R(INV_CS2_7);
   // k = 0: eu = 0, source 0.
   {
-    const double heq = wp0 * (1 + INV_CS2_7 * 0.0);
-    const double hk = hin[0];
-    hout[0] = (float)(hk - it * (hk - heq) + (double)w7(0) * 0.0);
+    const R heq = wp0 * (1 + c7 * R(0));
+    const R hk = hin[0];
+    hout[0] = (float)(hk - it * (hk - heq) + (R)w7(0) * R(0));
   }
-  const double u[3] = {uxi, uyi, uzi};
-  const double S[3] = {Sx, Sy, Sz};
+  const R u[3] = {uxi, uyi, uzi};
+  const R S[3] = {Sx, Sy, Sz};
 #pragma unroll
   for (int k = 1; k < Q7; k++) {
     const int a = (k - 1) >> 1;           // axis of direction k
     const bool pos = (k & 1) != 0;        // +1 for odd k
-    const double eu = pos ? u[a] : -u[a];
-    const double heq = wp1 * (1 + INV_CS2_7 * eu);
-    const double Fk = (double)w7(k) * (pos ? S[a] : -S[a]);
-    const double hk = hin[k];
+    const R eu = pos ? u[a] : -u[a];
+    const R heq = wp1 * (1 + c7 * eu);
+    const R Fk = (R)w7(k) * (pos ? S[a] : -S[a]);
+    const R hk = hin[k];
     hout[k] = (float)(hk - it * (hk - heq) + Fk);
   }
 }
 
 // collideHydro for one cell (src/sim.js:430-459).
-template <int K>
-__device__ __forceinline__ float collide_hydro_dir(const Derived &d, double sumF, double rhoPhase,
-                                                   double negU2half, double uF, double uxe,
-                                                   double uye, double uze, double fx, double fy,
-                                                   double fz, float fin) {
+template <int K, typename R>
+__device__ __forceinline__ float collide_hydro_dir(const Derived &d, R sumF, R rhoPhase,
+                                                   R negU2half, R uF, R uxe, R uye, R uze, R fx,
+                                                   R fy, R fz, float fin) {
   constexpr int ex = EX27[K], ey = EY27[K], ez = EZ27[K];
-  const double wk = w27(K);
-  const double eu = edot<ex, ey, ez>(uxe, uye, uze);
-  const double Qi = negU2half + eu * (INV_CS2_27 + INV_2CS4_27 * eu);
-  const double feq = wk * (sumF + rhoPhase * Qi);
-  const double ekF = edot<ex, ey, ez>(fx, fy, fz);
-  const double Sk = d.A * wk * (ekF * (1 + INV_CS2_27 * eu) - uF);
-  const double fk = fin;
-  return (float)(fk - d.invTau * (fk - feq) + Sk);
+  const R wk = (R)w27(K);
+  const R eu = edot<ex, ey, ez>(uxe, uye, uze);
+  const R Qi = negU2half + eu * (R(INV_CS2_27) + R(INV_2CS4_27) * eu);
+  const R feq = wk * (sumF + rhoPhase * Qi);
+  const R ekF = edot<ex, ey, ez>(fx, fy, fz);
+  const R Sk = R(d.A) * wk * (ekF * (1 + R(INV_CS2_27) * eu) - uF);
+  const R fk = fin;
+  return (float)(fk - R(d.invTau) * (fk - feq) + Sk);
 }
 
-template <int K>
+template <int K, typename R>
 struct HydroUnroll {
-  __device__ __forceinline__ static void run(const Derived &d, double sumF, double rhoPhase,
-                                             double negU2half, double uF, double uxe, double uye,
-                                             double uze, double fx, double fy, double fz,
+  __device__ __forceinline__ static void run(const Derived &d, R sumF, R rhoPhase, R negU2half,
+                                             R uF, R uxe, R uye, R uze, R fx, R fy, R fz,
                                              const float *fin, float *fout) {
-    HydroUnroll<K - 1>::run(d, sumF, rhoPhase, negU2half, uF, uxe, uye, uze, fx, fy, fz, fin, fout);
-    fout[K - 1] = collide_hydro_dir<K - 1>(d, sumF, rhoPhase, negU2half, uF, uxe, uye, uze, fx,
-                                           fy, fz, fin[K - 1]);
+    HydroUnroll<K - 1, R>::run(d, sumF, rhoPhase, negU2half, uF, uxe, uye, uze, fx, fy, fz, fin,
+                               fout);
+    fout[K - 1] = collide_hydro_dir<K - 1, R>(d, sumF, rhoPhase, negU2half, uF, uxe, uye, uze, fx,
+                                              fy, fz, fin[K - 1]);
   }
 };
-template <>
-struct HydroUnroll<0> {
-  __device__ __forceinline__ static void run(const Derived &, double, double, double, double,
-                                             double, double, double, double, double, double,
+template <typename R>
+struct HydroUnroll<0, R> {
+  __device__ __forceinline__ static void run(const Derived &, R, R, R, R, R, R, R, R, R, R,
                                              const float *, float *) {}
 };
 
-__device__ __forceinline__ void collide_hydro_cell(const Derived &d, double sumF, double p,
-                                                   double lap, double ggx, double ggy, double ggz,
-                                                   double bx, double by, double bz, double uxe,
-                                                   double uye, double uze, const float fin[Q27],
-                                                   float fout[Q27]) {
-  const double rhoPhase = d.rhoG + p * d.dRho;
-  const double mu = 4 * d.beta * p * (p - 1) * (2 * p - 1) - d.kappa * lap;
-  const double fx = bx + mu * ggx;
-  const double fy = by + mu * ggy + rhoPhase * d.gravity;
-  const double fz = bz + mu * ggz;
-  const double u2 = uxe * uxe + uye * uye + uze * uze;
-  const double negU2half = -1.5 * u2;
-  const double uF = uxe * fx + uye * fy + uze * fz;
-  HydroUnroll<Q27>::run(d, sumF, rhoPhase, negU2half, uF, uxe, uye, uze, fx, fy, fz, fin, fout);
+template <typename R>
+__device__ __forceinline__ void collide_hydro_cell(const Derived &d, R sumF, R p, R lap, R ggx,
+                                                   R ggy, R ggz, R bx, R by, R bz, R uxe, R uye,
+                                                   R uze, const float fin[Q27], float fout[Q27]) {
+  const R rhoPhase = R(d.rhoG) + p * R(d.dRho);
+  const R mu = 4 * R(d.beta) * p * (p - 1) * (2 * p - 1) - R(d.kappa) * lap;
+  const R fx = bx + mu * ggx;
+  const R fy = by + mu * ggy + rhoPhase * R(d.gravity);
+  const R fz = bz + mu * ggz;
+  const R u2 = uxe * uxe + uye * uye + uze * uze;
+  const R negU2half = R(-1.5) * u2;
+  const R uF = uxe * fx + uye * fy + uze * fz;
+  HydroUnroll<Q27, R>::run(d, sumF, rhoPhase, negU2half, uF, uxe, uye, uze, fx, fy, fz, fin, fout);
 }
 
 // Pull stream of the 27 populations with halfway bounce-back
 // (src/sim.js:509-543).  All source tags are read first and every load address
 // is chosen without branching (own opposite population at a non-fluid source),
 // so the 27 loads are in flight together; the moving-wall term and the momentum
-// exchange run only for solid links.  fk stays the unrounded double the
-// reference sums and exchanges; fp receives it rounded to float.
-__device__ __forceinline__ void pull27(const Geom &g, int64_t i, const float *__restrict__ f2,
+// exchange run only for solid links.  fk stays the unrounded value the
+// reference sums and exchanges; fp receives it rounded to float.  The momentum
+// exchange accumulates in FP64 whatever the arithmetic type.
+template <int KIND>
+__device__ __forceinline__ void pull27(const Geom &g, int64_t i,
+                                       const typename Codec<KIND>::T *__restrict__ f2,
                                        const uint8_t *__restrict__ tag,
                                        const float *__restrict__ usx,
                                        const float *__restrict__ usy,
-                                       const float *__restrict__ usz, double rhoPhaseOld,
-                                       bool exchange, int x, int y, int z, double cx, double cy,
-                                       double cz, double mom[6], bool &linked, float fp[Q27],
-                                       double &pSumF, double &mx, double &my, double &mz,
-                                       uint32_t &nonfluid) {
+                                       const float *__restrict__ usz,
+                                       typename Codec<KIND>::R rhoPhaseOld, bool exchange, int x,
+                                       int y, int z, double cx, double cy, double cz,
+                                       double mom[6], float fp[Q27],
+                                       typename Codec<KIND>::R &pSumF,
+                                       typename Codec<KIND>::R &mx, typename Codec<KIND>::R &my,
+                                       typename Codec<KIND>::R &mz, uint32_t &nonfluid) {
+  using C = Codec<KIND>;
+  using R = typename C::R;
   nonfluid = 0;
   uint32_t solid = 0;
 #pragma unroll
@@ -218,24 +225,23 @@ __device__ __forceinline__ void pull27(const Geom &g, int64_t i, const float *__
   for (int k = 0; k < Q27; k++) {
     const int64_t s = i - ((int64_t)EZ27[k] * g.nxny + (int64_t)EY27[k] * g.nx + EX27[k]);
     const int64_t a = ((nonfluid >> k) & 1u) ? opp27(k) * g.S + i : k * g.S + s;
-    raw[k] = f2[a];
+    raw[k] = C::ld(f2, a, w27(k));  // w27(opp(k)) == w27(k)
   }
   pSumF = 0;
   mx = my = mz = 0;
 #pragma unroll
   for (int k = 0; k < Q27; k++) {
-    double fk = (double)raw[k];
+    R fk = (R)raw[k];
     if ((solid >> k) & 1u) {
       const int ex = EX27[k], ey = EY27[k], ez = EZ27[k];
       const int64_t s = i - ((int64_t)ez * g.nxny + (int64_t)ey * g.nx + ex);
-      const double fopp = fk;
-      const double eu = (double)ex * (double)usx[s] + (double)ey * (double)usy[s] +
-                        (double)ez * (double)usz[s];
-      const double corr = 2 * (double)w27(k) * rhoPhaseOld * eu * INV_CS2_27;
+      const R fopp = fk;
+      const R eu = (R)ex * (R)usx[s] + (R)ey * (R)usy[s] + (R)ez * (R)usz[s];
+      const R corr = 2 * (R)w27(k) * rhoPhaseOld * eu * R(INV_CS2_27);
       fk = fopp + corr;
       if (exchange) {
         // applyImpulseAtCell(x - ex, y - ey, z - ez, -m ex, -m ey, -m ez)
-        const double m = fopp + fk;
+        const double m = (double)fopp + (double)fk;
         const double px = -m * ex, py = -m * ey, pz = -m * ez;
         const double rx = (x - ex) + 0.5 - cx;
         const double ry = (y - ey) + 0.5 - cy;
@@ -246,7 +252,6 @@ __device__ __forceinline__ void pull27(const Geom &g, int64_t i, const float *__
         mom[3] += ry * pz - rz * py;
         mom[4] += rz * px - rx * pz;
         mom[5] += rx * py - ry * px;
-        linked = true;
       }
     }
     fp[k] = (float)fk;
@@ -262,6 +267,27 @@ __device__ __forceinline__ void pull27(const Geom &g, int64_t i, const float *__
   }
 }
 
+// Pull of the 7 phase populations (src/sim.js:376-397): from the source cell,
+// or the own opposite population when the source is not fluid (D3Q7 directions
+// 1..6 are D3Q27 directions 1..6, so the D3Q27 source mask serves).  A stored h
+// carries the phi of the cell it is stored at as its baseline.
+template <int KIND>
+__device__ __forceinline__ void pull7(const Geom &g, int64_t i,
+                                      const typename Codec<KIND>::T *__restrict__ h2,
+                                      const float *__restrict__ phi, uint32_t nonfluid,
+                                      float hp[Q7]) {
+  using C = Codec<KIND>;
+#pragma unroll
+  for (int k = 0; k < Q7; k++) {
+    const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
+    const bool bounce = (nonfluid >> k) & 1u;
+    const int64_t a = bounce ? opp7(k) * g.S + i : k * g.S + s;
+    float base = 0.0f;
+    if (KIND != ST_F32) base = w7(k) * phi[bounce ? i : s];
+    hp[k] = C::ld(h2, a, base);
+  }
+}
+
 // Sum of a double over the 64 lanes of a wave by a fixed butterfly; every lane
 // receives the same total.
 __device__ __forceinline__ double wave_sum(double v) {
@@ -272,8 +298,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Link momentum of one tile (6 doubles), summed in a fixed order (butterfly
 // within each wave, waves in order) and stored at partials[tile].  Every tile
-// that intersects the band box stores its sum, zero or not, so the body update
-// can read exactly the band's tiles without a list.
+// of the band stores its sum, zero or not, so the body update can read exactly
+// the band's tiles without a list.
 __device__ __forceinline__ void tile_momentum(double mom[6], double *partials, int64_t tile) {
   __shared__ double red[4][6];
   const int lane = threadIdx.x;  // 0..63
@@ -292,14 +318,14 @@ __device__ __forceinline__ void tile_momentum(double mom[6], double *partials, i
   }
 }
 
-
 // ---------------------------------------------------------------- scenes
 
 // One simulation of a scene set.  Buffer pairs are stored by step parity: a
 // step of parity q reads the post-collision populations f[q] (written by the
 // previous step) and writes f[q ^ 1]; it reads phi(t) from phi[q] and writes
 // phi(t+1) to phi[q ^ 1].  The first collide of a call reads the reference's f
-// from f[1] and writes f[0].
+// from f[1] and writes f[0].  The population pointers address elements of the
+// scene set's storage codec (float, or 16-bit deviations).
 struct Scene {
   Geom g;
   TileMap tm;
@@ -321,12 +347,17 @@ struct Scene {
   int bgx, bgy, bgz;        // band blocks of 64x4 cells (covers the largest band box)
 };
 
-// Buffer of parity q (0 or 1) of a pair.  Both pointers are loaded and one is
-// selected, so the load does not wait on q.
-template <typename T>
-__device__ __forceinline__ T *par(T *const pair[2], int q) {
-  T *a = pair[0];
-  T *b = pair[1];
+// Buffer of parity q (0 or 1) of a pair, as codec elements.  Both pointers are
+// loaded and one is selected, so the load does not wait on q.
+template <int KIND>
+__device__ __forceinline__ typename Codec<KIND>::T *pop(float *const pair[2], int q) {
+  float *a = pair[0];
+  float *b = pair[1];
+  return reinterpret_cast<typename Codec<KIND>::T *>(q ? b : a);
+}
+__device__ __forceinline__ float *par(float *const pair[2], int q) {
+  float *a = pair[0];
+  float *b = pair[1];
   return q ? b : a;
 }
 
@@ -340,7 +371,6 @@ __device__ __forceinline__ int find_scene(const int *__restrict__ off, int n, in
   }
   return lo;
 }
-
 
 // Cell of block lb of a 64x4-block launch over the whole grid or over the band
 // box; false when outside.
@@ -362,18 +392,8 @@ __device__ __forceinline__ bool block_cell(const Scene &sc, int lb, int &x, int 
 // Every kernel comes in two launch forms: km_* over a scene table (a block finds
 // its scene by binary search in the prefix table off of block offsets) and k1_*
 // for one scene passed by value, whose descriptor arrives with the kernel
-// arguments and costs no dependent load at block start.
-#define LBM_SCENE_KERNELS(NAME, BODY, TPARAMS, TARGS, ...)                               \
-  TPARAMS __global__ __launch_bounds__(256) void km_##NAME(                             \
-      const Scene *__restrict__ scenes, const int *__restrict__ off, int nscenes,      \
-      LBM_PARAMS_##NAME) {                                                               \
-    const int si = find_scene(off, nscenes, blockIdx.x);                                 \
-    BODY TARGS(scenes[si], blockIdx.x - off[si], __VA_ARGS__);                           \
-  }                                                                                      \
-  TPARAMS __global__ __launch_bounds__(256) void k1_##NAME(const Scene sc,               \
-                                                           LBM_PARAMS_##NAME) {         \
-    BODY TARGS(sc, blockIdx.x, __VA_ARGS__);                                             \
-  }
+// arguments and costs no dependent load at block start.  KIND selects the
+// population codec of every scene of the launch.
 
 // ---------------------------------------------------------------- retag
 
@@ -381,7 +401,7 @@ __device__ __forceinline__ bool block_cell(const Scene &sc, int lb, int &x, int 
 // (later steps).  No phi snapshot: in the fused form phiPrev is the other phi
 // buffer.
 template <bool BAND>
-__device__ __forceinline__ void retag_body(const Scene &sc, int lb, int) {
+__device__ __forceinline__ void retag_body(const Scene &sc, int lb) {
   int x, y, z;
   if (!block_cell<BAND>(sc, lb, x, y, z)) return;
   lbm_sphere b{};
@@ -389,13 +409,22 @@ __device__ __forceinline__ void retag_body(const Scene &sc, int lb, int) {
   retag_cell(sc.g, sc.has_solid ? 1 : 0, b, nullptr, nullptr, sc.tag, sc.tagPrev, nullptr,
              nullptr, 0, sc.usx, sc.usy, sc.usz, sc.ctrl, sc.fresh, sc.dead, sc.list_cap, x, y, z);
 }
-#define LBM_PARAMS_retag int unused
-LBM_SCENE_KERNELS(retag, retag_body, template <bool BAND>, <BAND>, unused)
+template <bool BAND>
+__global__ __launch_bounds__(256) void km_retag(const Scene *__restrict__ scenes,
+                                                const int *__restrict__ off, int nscenes) {
+  const int si = find_scene(off, nscenes, blockIdx.x);
+  retag_body<BAND>(scenes[si], blockIdx.x - off[si]);
+}
+template <bool BAND>
+__global__ __launch_bounds__(256) void k1_retag(const Scene sc) {
+  retag_body<BAND>(sc, blockIdx.x);
+}
 
 // Sorts both lists of the scene's last retag, then runs handleFreshCells; one
 // workgroup per scene.  Upcoming step p: phi is phi[p]; the post-stream
 // populations of fresh cells go to the reference's f (f[1]) before the first
 // step of a call, and in place into f[p] for band updates.
+template <int KIND>
 __device__ __forceinline__ void sort_fresh_body(const Scene &sc, int p, int prologue) {
   __shared__ int32_t sh[kSortLds];
   int nf = sc.ctrl->n_fresh, nd = sc.ctrl->n_dead;
@@ -405,24 +434,30 @@ __device__ __forceinline__ void sort_fresh_body(const Scene &sc, int p, int prol
   sort_list_block(sc.dead, nd, sh);
   const int fi = prologue ? 1 : (p & 1);
   float *phi = par(sc.phi, p & 1);
-  fresh_block(sc.g, sc.d.rhoG, sc.rhoL, sc.fresh, nf, sc.level_scratch, sc.tag, phi, phi, phi,
-              sc.rho, sc.ux, sc.uy, sc.uz, par(sc.f, fi), par(sc.h, fi));
+  fresh_block<KIND>(sc.g, sc.d.rhoG, sc.rhoL, sc.fresh, nf, sc.level_scratch, sc.tag, phi, phi,
+                    phi, sc.rho, sc.ux, sc.uy, sc.uz, pop<KIND>(sc.f, fi), pop<KIND>(sc.h, fi));
 }
+template <int KIND>
 __global__ __launch_bounds__(1024) void km_sort_fresh(const Scene *__restrict__ scenes, int p,
                                                       int prologue) {
-  sort_fresh_body(scenes[blockIdx.x], p, prologue);
+  sort_fresh_body<KIND>(scenes[blockIdx.x], p, prologue);
 }
+template <int KIND>
 __global__ __launch_bounds__(1024) void k1_sort_fresh(const Scene sc, int p, int prologue) {
-  sort_fresh_body(sc, p, prologue);
+  sort_fresh_body<KIND>(sc, p, prologue);
 }
 
 // ---------------------------------------------------------------- collide
 
 // Gradients + collide for the fluid cells of the whole grid (first step of a
 // call: post-stream populations in f[1] -> post-collision f[0], the external
-// force if armed) or of the band box before step p (in place in f[p]).
-template <bool BAND>
+// force if armed) or of the band box before step p (in place in f[p]).  The h
+// baseline is phi of the same cell before and after: the collide keeps phi.
+template <bool BAND, int KIND>
 __device__ __forceinline__ void collide_body(const Scene &sc, int lb, int p) {
+  using C = Codec<KIND>;
+  using R = typename C::R;
+  using T = typename C::T;
   const Geom &g = sc.g;
   int x, y, z;
   if (!block_cell<BAND>(sc, lb, x, y, z)) return;
@@ -430,35 +465,45 @@ __device__ __forceinline__ void collide_body(const Scene &sc, int lb, int p) {
   const int64_t i = cell_index(g, x, y, z);
   if (sc.tag[i] != TAG_FLUID) return;
   p &= 1;
-  const float *fi = BAND ? par(sc.f, p) : sc.f[1];
-  float *fo = BAND ? par(sc.f, p) : sc.f[0];
-  const float *hi = BAND ? par(sc.h, p) : sc.h[1];
-  float *ho = BAND ? par(sc.h, p) : sc.h[0];
+  const T *fi = BAND ? pop<KIND>(sc.f, p) : pop<KIND>(sc.f, 1);
+  T *fo = BAND ? pop<KIND>(sc.f, p) : pop<KIND>(sc.f, 0);
+  const T *hi = BAND ? pop<KIND>(sc.h, p) : pop<KIND>(sc.h, 1);
+  T *ho = BAND ? pop<KIND>(sc.h, p) : pop<KIND>(sc.h, 0);
   const float *phi = BAND ? par(sc.phi, p) : sc.phi[0];
-  const double pc = phi[i];
+  const float phc = phi[i];
+  const R pc = phc;
   float gx1, gy1, gz1, lap1;
-  phi_grad(g, phi, i, pc, gx1, gy1, gz1, lap1);
-  const double uxe = sc.ux[i], uye = sc.uy[i], uze = sc.uz[i];
+  phi_grad<R>(g, phi, i, pc, gx1, gy1, gz1, lap1);
+  const R uxe = sc.ux[i], uye = sc.uy[i], uze = sc.uz[i];
   float hin[Q7], hc[Q7];
 #pragma unroll
-  for (int k = 0; k < Q7; k++) hin[k] = hi[k * g.S + i];
-  collide_phase_cell(sc.d, pc, uxe, uye, uze, (double)gx1, (double)gy1, (double)gz1, hin, hc);
+  for (int k = 0; k < Q7; k++) hin[k] = C::ld(hi, k * g.S + i, w7(k) * phc);
+  collide_phase_cell<R>(sc.d, pc, uxe, uye, uze, (R)gx1, (R)gy1, (R)gz1, hin, hc);
 #pragma unroll
-  for (int k = 0; k < Q7; k++) ho[k * g.S + i] = hc[k];
+  for (int k = 0; k < Q7; k++) C::st(ho, k * g.S + i, hc[k], w7(k) * phc);
   float fin[Q27], fc[Q27];
 #pragma unroll
-  for (int k = 0; k < Q27; k++) fin[k] = fi[k * g.S + i];
+  for (int k = 0; k < Q27; k++) fin[k] = C::ld(fi, k * g.S + i, w27(k));
   const bool bf = !BAND && sc.use_bf;
-  const double bx = bf ? (double)sc.bfx[i] : 0.0;
-  const double by = bf ? (double)sc.bfy[i] : 0.0;
-  const double bz = bf ? (double)sc.bfz[i] : 0.0;
-  collide_hydro_cell(sc.d, (double)sc.rho[i], pc, (double)lap1, (double)gx1, (double)gy1,
-                     (double)gz1, bx, by, bz, uxe, uye, uze, fin, fc);
+  const R bx = bf ? (R)sc.bfx[i] : R(0);
+  const R by = bf ? (R)sc.bfy[i] : R(0);
+  const R bz = bf ? (R)sc.bfz[i] : R(0);
+  collide_hydro_cell<R>(sc.d, (R)sc.rho[i], pc, (R)lap1, (R)gx1, (R)gy1, (R)gz1, bx, by, bz, uxe,
+                        uye, uze, fin, fc);
 #pragma unroll
-  for (int k = 0; k < Q27; k++) fo[k * g.S + i] = fc[k];
+  for (int k = 0; k < Q27; k++) C::st(fo, k * g.S + i, fc[k], w27(k));
 }
-#define LBM_PARAMS_collide int p
-LBM_SCENE_KERNELS(collide, collide_body, template <bool BAND>, <BAND>, p)
+template <bool BAND, int KIND>
+__global__ __launch_bounds__(256) void km_collide(const Scene *__restrict__ scenes,
+                                                  const int *__restrict__ off, int nscenes,
+                                                  int p) {
+  const int si = find_scene(off, nscenes, blockIdx.x);
+  collide_body<BAND, KIND>(scenes[si], blockIdx.x - off[si], p);
+}
+template <bool BAND, int KIND>
+__global__ __launch_bounds__(256) void k1_collide(const Scene sc, int p) {
+  collide_body<BAND, KIND>(sc, blockIdx.x, p);
+}
 
 // ---------------------------------------------------------------- phase stream
 
@@ -466,17 +511,20 @@ LBM_SCENE_KERNELS(collide, collide_body, template <bool BAND>, <BAND>, p)
 // (src/sim.js:376-397, 549-552); other cells keep phi (the reference never
 // updates them).  Source tags are read first and every load address is chosen
 // without branching, so the seven loads are in flight together.
+template <int KIND>
 __device__ __forceinline__ void phase_stream_body(const Scene &sc, int lb, int q) {
+  using R = typename Codec<KIND>::R;
   const Geom &g = sc.g;
-  int tile, z, c0, x, y;
-  if (!map_tile(sc.tm, lb, tile, z, c0)) return;
-  if (!tile_cell(sc.tm, c0, x, y)) return;
+  int tile, x, y, z;
+  if (!map_tile(sc.tm.T, sc.tm.T8, lb, tile)) return;
+  const int64_t i = ((int64_t)tile << kTileShift) + threadIdx.y * 64 + threadIdx.x;
+  if (i >= g.n) return;
+  cell_xyz(g.nx, g.nxny, i, x, y, z);
   q &= 1;
-  const float *h2 = par(sc.h, q);
+  const typename Codec<KIND>::T *h2 = pop<KIND>(sc.h, q);
   const float *phi = par(sc.phi, q);
   float *phi_next = par(sc.phi, q ^ 1);
   const uint8_t *tag = sc.tag;
-  const int64_t i = cell_index(g, x, y, z);
   if (tag[i] != TAG_FLUID || !interior(g, x, y, z)) {
     phi_next[i] = phi[i];
     return;
@@ -488,20 +536,24 @@ __device__ __forceinline__ void phase_stream_body(const Scene &sc, int lb, int q
     nonfluid |= (uint32_t)(tag[s] != TAG_FLUID) << k;
   }
   float hv[Q7];
+  pull7<KIND>(g, i, h2, phi, nonfluid, hv);
+  R phiSum = 0;
 #pragma unroll
-  for (int k = 0; k < Q7; k++) {
-    const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
-    const int64_t a = ((nonfluid >> k) & 1u) ? opp7(k) * g.S + i : k * g.S + s;
-    hv[k] = h2[a];
-  }
-  double phiSum = 0;
-#pragma unroll
-  for (int k = 0; k < Q7; k++) phiSum += (double)hv[k];
-  const double pp = phiSum < 0 ? 0 : phiSum > 1 ? 1 : phiSum;
+  for (int k = 0; k < Q7; k++) phiSum += (R)hv[k];
+  const R pp = phiSum < 0 ? R(0) : phiSum > 1 ? R(1) : phiSum;
   phi_next[i] = (float)pp;
 }
-#define LBM_PARAMS_phase_stream int q
-LBM_SCENE_KERNELS(phase_stream, phase_stream_body, , , q)
+template <int KIND>
+__global__ __launch_bounds__(256) void km_phase_stream(const Scene *__restrict__ scenes,
+                                                       const int *__restrict__ off, int nscenes,
+                                                       int q) {
+  const int si = find_scene(off, nscenes, blockIdx.x);
+  phase_stream_body<KIND>(scenes[si], blockIdx.x - off[si], q);
+}
+template <int KIND>
+__global__ __launch_bounds__(256) void k1_phase_stream(const Scene sc, int q) {
+  phase_stream_body<KIND>(sc, blockIdx.x, q);
+}
 
 // ---------------------------------------------------------------- stream + collide
 
@@ -511,35 +563,37 @@ LBM_SCENE_KERNELS(phase_stream, phase_stream_body, , , q)
 // leaves the state exactly as the reference's step does, and phi(t+1) is
 // written here.  first: the first step of the call (the armed external force
 // applies to its macroscopic update).
-template <bool COLLIDE>
+template <bool COLLIDE, int KIND>
 __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int q, int first) {
+  using C = Codec<KIND>;
+  using R = typename C::R;
+  using T = typename C::T;
   const Geom &g = sc.g;
   const Derived &d = sc.d;
-  int tile, z, c0, x, y;
-  if (!map_tile(sc.tm, lb, tile, z, c0)) return;
-  const bool in_plane = tile_cell(sc.tm, c0, x, y);
+  int tile, x, y, z;
+  if (!map_tile(sc.tm.T, sc.tm.T8, lb, tile)) return;
+  const int64_t i = ((int64_t)tile << kTileShift) + threadIdx.y * 64 + threadIdx.x;
+  const bool in_grid = i < g.n;
+  cell_xyz(g.nx, g.nxny, i, x, y, z);
   q &= 1;
-  const float *__restrict__ f2 = par(sc.f, q);
-  const float *__restrict__ h2 = par(sc.h, q);
-  float *__restrict__ fo = par(sc.f, q ^ 1);
-  float *__restrict__ ho = par(sc.h, q ^ 1);
+  const T *__restrict__ f2 = pop<KIND>(sc.f, q);
+  const T *__restrict__ h2 = pop<KIND>(sc.h, q);
+  T *__restrict__ fo = pop<KIND>(sc.f, q ^ 1);
+  T *__restrict__ ho = pop<KIND>(sc.h, q ^ 1);
   const float *__restrict__ phi = par(sc.phi, q);
   float *__restrict__ phi_next = par(sc.phi, q ^ 1);
   const uint8_t *__restrict__ tag = sc.tag;
   const Ctrl *__restrict__ ctrl = sc.ctrl;
   double mom[6] = {0, 0, 0, 0, 0, 0};
-  bool linked = false;
   int box[6];
 #pragma unroll
   for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
-  // Link momentum can only arise next to the sphere, which lies inside the band:
-  // the tiles of the band's planes between its first and last cell store their
-  // sums (zero or not), and the body update reads exactly those.
+  // Link momentum can only arise next to the sphere, inside the band box: the
+  // tiles from the one holding the box's first cell to the one holding its
+  // last store their sums (zero or not), and the body update reads those.
   int tlo, thi;
-  box_tiles(sc.tm, box, tlo, thi);
-  const int lt = tile - z * sc.tm.tpp;
-  const bool tile_in_band = z >= box[4] && z <= box[5] && lt >= tlo && lt <= thi;
-  const bool exchange = sc.has_solid != 0 && tile_in_band;
+  box_tile_range(sc.tm, box, tlo, thi);
+  const bool exchange = sc.has_solid != 0 && tile >= tlo && tile <= thi;
   const int use_bf = first && sc.use_bf;
   double cx = 0, cy = 0, cz = 0;
   if (exchange) {
@@ -547,51 +601,43 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
     cy = ctrl->sphere.cy;
     cz = ctrl->sphere.cz;
   }
-  if (in_plane) {
-    const int64_t i = cell_index(g, x, y, z);
+  if (in_grid) {
     const bool fluid = tag[i] == TAG_FLUID && interior(g, x, y, z);
     if (!fluid) {
       if (!COLLIDE) phi_next[i] = phi[i];
     } else {
-      const double phiOld = phi[i];
-      const double rhoPhaseOld = d.rhoG + phiOld * d.dRho;
+      const R phiOld = phi[i];
+      const R rhoPhaseOld = R(d.rhoG) + phiOld * R(d.dRho);
       float gxo, gyo, gzo, lapo;
-      phi_grad(g, phi, i, phiOld, gxo, gyo, gzo, lapo);
+      phi_grad<R>(g, phi, i, phiOld, gxo, gyo, gzo, lapo);
       float fp[Q27];
-      double pSumF, mx, my, mz;
+      R pSumF, mx, my, mz;
       uint32_t nonfluid;
-      pull27(g, i, f2, tag, sc.usx, sc.usy, sc.usz, rhoPhaseOld, exchange, x, y, z, cx, cy, cz,
-             mom, linked, fp, pSumF, mx, my, mz, nonfluid);
-      const double rhoN = pSumF > 0.01 ? pSumF : 0.01;
-      // D3Q7 directions 1..6 are D3Q27 directions 1..6, so the same source mask
-      // selects their bounce-back.
+      pull27<KIND>(g, i, f2, tag, sc.usx, sc.usy, sc.usz, rhoPhaseOld, exchange, x, y, z, cx, cy,
+                   cz, mom, fp, pSumF, mx, my, mz, nonfluid);
+      const R rhoN = pSumF > R(0.01) ? pSumF : R(0.01);
       float hp[Q7];
+      pull7<KIND>(g, i, h2, phi, nonfluid, hp);
+      R phiSum = 0;
 #pragma unroll
-      for (int k = 0; k < Q7; k++) {
-        const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
-        const int64_t a = ((nonfluid >> k) & 1u) ? opp7(k) * g.S + i : k * g.S + s;
-        hp[k] = h2[a];
-      }
-      double phiSum = 0;
-#pragma unroll
-      for (int k = 0; k < Q7; k++) phiSum += (double)hp[k];
-      const double p = phiSum < 0 ? 0 : phiSum > 1 ? 1 : phiSum;
-      const double rhoPhaseNew = d.rhoG + p * d.dRho;
-      const double mu = 4 * d.beta * p * (p - 1) * (2 * p - 1) - d.kappa * (double)lapo;
-      const double bxf = use_bf ? (double)sc.bfx[i] : 0.0;
-      const double byf = use_bf ? (double)sc.bfy[i] : 0.0;
-      const double bzf = use_bf ? (double)sc.bfz[i] : 0.0;
-      const double fx = bxf + mu * (double)gxo;
-      const double fy = byf + mu * (double)gyo + rhoPhaseNew * d.gravity;
-      const double fz = bzf + mu * (double)gzo;
-      const double invRho = 1 / rhoPhaseNew;
-      double uxn = (mx + 0.5 * fx) * invRho;
-      double uyn = (my + 0.5 * fy) * invRho;
-      double uzn = (mz + 0.5 * fz) * invRho;
-      const double u2 = uxn * uxn + uyn * uyn + uzn * uzn;
-      const double UMAX = 0.2, UMAX2 = UMAX * UMAX;
+      for (int k = 0; k < Q7; k++) phiSum += (R)hp[k];
+      const R p = phiSum < 0 ? R(0) : phiSum > 1 ? R(1) : phiSum;
+      const R rhoPhaseNew = R(d.rhoG) + p * R(d.dRho);
+      const R mu = 4 * R(d.beta) * p * (p - 1) * (2 * p - 1) - R(d.kappa) * (R)lapo;
+      const R bxf = use_bf ? (R)sc.bfx[i] : R(0);
+      const R byf = use_bf ? (R)sc.bfy[i] : R(0);
+      const R bzf = use_bf ? (R)sc.bfz[i] : R(0);
+      const R fx = bxf + mu * (R)gxo;
+      const R fy = byf + mu * (R)gyo + rhoPhaseNew * R(d.gravity);
+      const R fz = bzf + mu * (R)gzo;
+      const R invRho = 1 / rhoPhaseNew;
+      R uxn = (mx + R(0.5) * fx) * invRho;
+      R uyn = (my + R(0.5) * fy) * invRho;
+      R uzn = (mz + R(0.5) * fz) * invRho;
+      const R u2 = uxn * uxn + uyn * uyn + uzn * uzn;
+      const R UMAX = R(0.2), UMAX2 = UMAX * UMAX;
       if (u2 > UMAX2) {
-        const double scl = UMAX / sqrt(u2);
+        const R scl = UMAX / sqrt(u2);
         uxn *= scl;
         uyn *= scl;
         uzn *= scl;
@@ -615,43 +661,113 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
           // collide(t+1) with the new macroscopic values (rounded to float as
           // the reference stores them) and the gradients of phi(t+1).
           float gx1, gy1, gz1, lap1;
-          phi_grad(g, phi_next, i, (double)phif, gx1, gy1, gz1, lap1);
+          phi_grad<R>(g, phi_next, i, (R)phif, gx1, gy1, gz1, lap1);
           float hc[Q7];
-          collide_phase_cell(d, (double)phif, (double)uxf, (double)uyf, (double)uzf, (double)gx1,
-                             (double)gy1, (double)gz1, hp, hc);
+          collide_phase_cell<R>(d, (R)phif, (R)uxf, (R)uyf, (R)uzf, (R)gx1, (R)gy1, (R)gz1, hp,
+                                hc);
 #pragma unroll
-          for (int k = 0; k < Q7; k++) ho[k * g.S + i] = hc[k];
+          for (int k = 0; k < Q7; k++) C::st(ho, k * g.S + i, hc[k], w7(k) * phif);
           float fc[Q27];
-          collide_hydro_cell(d, (double)rhof, (double)phif, (double)lap1, (double)gx1,
-                             (double)gy1, (double)gz1, 0.0, 0.0, 0.0, (double)uxf, (double)uyf,
-                             (double)uzf, fp, fc);
+          collide_hydro_cell<R>(d, (R)rhof, (R)phif, (R)lap1, (R)gx1, (R)gy1, (R)gz1, R(0), R(0),
+                                R(0), (R)uxf, (R)uyf, (R)uzf, fp, fc);
 #pragma unroll
-          for (int k = 0; k < Q27; k++) fo[k * g.S + i] = fc[k];
+          for (int k = 0; k < Q27; k++) C::st(fo, k * g.S + i, fc[k], w27(k));
         }
       } else {
         phi_next[i] = phif;
       }
       if (store_stream) {
 #pragma unroll
-        for (int k = 0; k < Q27; k++) fo[k * g.S + i] = fp[k];
+        for (int k = 0; k < Q27; k++) C::st(fo, k * g.S + i, fp[k], w27(k));
 #pragma unroll
-        for (int k = 0; k < Q7; k++) ho[k * g.S + i] = hp[k];
+        for (int k = 0; k < Q7; k++) C::st(ho, k * g.S + i, hp[k], w7(k) * phif);
       }
     }
   }
-  (void)linked;
   if (exchange) tile_momentum(mom, sc.partials, tile);
 }
-#define LBM_PARAMS_stream_collide int q, int first
-LBM_SCENE_KERNELS(stream_collide, stream_collide_body, template <bool COLLIDE>, <COLLIDE>, q,
-                  first)
+template <bool COLLIDE, int KIND>
+__global__ __launch_bounds__(256) void km_stream_collide(const Scene *__restrict__ scenes,
+                                                         const int *__restrict__ off, int nscenes,
+                                                         int q, int first) {
+  const int si = find_scene(off, nscenes, blockIdx.x);
+  stream_collide_body<COLLIDE, KIND>(scenes[si], blockIdx.x - off[si], q, first);
+}
+template <bool COLLIDE, int KIND>
+__global__ __launch_bounds__(256) void k1_stream_collide(const Scene sc, int q, int first) {
+  stream_collide_body<COLLIDE, KIND>(sc, blockIdx.x, q, first);
+}
+// ---------------------------------------------------------------- storage conversion
+
+// Float32 populations (the reference's f and h) to a 16-bit codec, and back.
+// The h baseline is phi of the cell (the phi the populations belong to).
+template <int KIND>
+__global__ __launch_bounds__(256) void k_encode(int64_t n, int64_t S, const float *__restrict__ f,
+                                                const float *__restrict__ h,
+                                                const float *__restrict__ phi,
+                                                typename Codec<KIND>::T *__restrict__ cf,
+                                                typename Codec<KIND>::T *__restrict__ ch) {
+  using C = Codec<KIND>;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float pc = phi[i];
+#pragma unroll
+  for (int k = 0; k < Q27; k++) C::st(cf, k * S + i, f[k * S + i], w27(k));
+#pragma unroll
+  for (int k = 0; k < Q7; k++) C::st(ch, k * S + i, h[k * S + i], w7(k) * pc);
+}
+template <int KIND>
+__global__ __launch_bounds__(256) void k_decode(int64_t n, int64_t S,
+                                                const typename Codec<KIND>::T *__restrict__ cf,
+                                                const typename Codec<KIND>::T *__restrict__ ch,
+                                                const float *__restrict__ phi,
+                                                float *__restrict__ f, float *__restrict__ h) {
+  using C = Codec<KIND>;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float pc = phi[i];
+#pragma unroll
+  for (int k = 0; k < Q27; k++) f[k * S + i] = C::ld(cf, k * S + i, w27(k));
+#pragma unroll
+  for (int k = 0; k < Q7; k++) h[k * S + i] = C::ld(ch, k * S + i, w7(k) * pc);
+}
+
+// Global mass check: sums over interior fluid cells of phi and of rho (the
+// pressure proxy sum f), in FP64, one partial pair per 256-cell block; the host
+// adds the partials in block order, so the result is deterministic.
+__global__ __launch_bounds__(256) void k_mass_partials(int64_t n, Geom g,
+                                                       const uint8_t *__restrict__ tag,
+                                                       const float *__restrict__ phi,
+                                                       const float *__restrict__ rho,
+                                                       double *__restrict__ partials) {
+  __shared__ double red[4][2];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double a = 0, b = 0;
+  if (i < n && tag[i] == TAG_FLUID) {
+    a = phi[i];
+    b = rho[i];
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[wave][0] = a;
+    red[wave][1] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = ((red[0][0] + red[1][0]) + red[2][0]) + red[3][0];
+    partials[2 * blockIdx.x + 1] = ((red[0][1] + red[1][1]) + red[2][1]) + red[3][1];
+  }
+}
+
 
 // ---------------------------------------------------------------- body
 
 // Body update after stream(t) of step q, one 512-thread workgroup per scene:
-//   F/T += link momentum of the band's tiles (in each plane of this step's
-//          band box, the tiles from its first to its last cell), in ascending
-//          tile order (contiguous chunks per thread, then waves in order);
+//   F/T += link momentum of the tiles from the one holding this step's band
+//          box's first cell to the one holding its last, in ascending tile
+//          order (contiguous chunks per thread, then waves in order);
 //   F/T += dead-cell impulses rho_phase(phi) u at the cells that turned solid at
 //          this step's retag (src/solid.js:87-101), in the same chunked order;
 //          a dead cell keeps the phi and u it had when it turned solid, which
@@ -683,10 +799,8 @@ __device__ __forceinline__ void body_body(const Scene &sc, int q, int check_band
 #pragma unroll
   for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
   int tlo, thi;
-  box_tiles(tm, box, tlo, thi);
-  const int nbt = thi - tlo + 1;
-  const int bz0 = box[4], nbz = box[5] - box[4] + 1;
-  const int ntiles = nbt * nbz;
+  box_tile_range(tm, box, tlo, thi);
+  const int ntiles = thi - tlo + 1;
   double acc[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) acc[k] = 0;
@@ -694,8 +808,7 @@ __device__ __forceinline__ void body_body(const Scene &sc, int q, int check_band
     const int C = (ntiles + kBodyThreads - 1) / kBodyThreads;
     const int lo = tid * C, hi = min(ntiles, lo + C);
     for (int c = lo; c < hi; c++) {
-      const int tz_ = c / nbt;
-      const int64_t t = (int64_t)(bz0 + tz_) * tm.tpp + tlo + (c - tz_ * nbt);
+      const int64_t t = tlo + c;
 #pragma unroll
       for (int k = 0; k < 6; k++) acc[k] += sc.partials[t * 6 + k];
     }

@@ -8,6 +8,7 @@
 // (ceil(nx/64), ceil(ny/4), nz).
 #pragma once
 
+#include "codec.h"
 #include "device_common.h"
 
 namespace lbm {
@@ -218,10 +219,13 @@ __device__ __forceinline__ int lower_bound_i32(const int32_t *a, int n, int32_t 
 // phi_fb: phiPrev read for the no-neighbour fallback; phi_w1/phi_w2: the two
 // writes of phi[i] = phiPrev[i] = p (they may alias); neighbours read phi_w1.
 // f and h receive the reseeded populations.
+template <int KIND>
 __device__ void fresh_block(Geom g, double rhoG, double rhoL, const int32_t *__restrict__ list,
                             int count, int *level_scratch, const uint8_t *__restrict__ tag,
                             const float *phi_fb, float *phi_w1, float *phi_w2, float *rho,
-                            float *ux, float *uy, float *uz, float *f, float *h) {
+                            float *ux, float *uy, float *uz, typename Codec<KIND>::T *f,
+                            typename Codec<KIND>::T *h) {
+  using C = Codec<KIND>;
   __shared__ int sh_level[4096];
   __shared__ int sh_flag;
   if (count == 0) return;
@@ -289,8 +293,9 @@ __device__ void fresh_block(Geom g, double rhoG, double rhoL, const int32_t *__r
       const double vx = pcnt > 0 ? vxs / pcnt : 0;
       const double vy = pcnt > 0 ? vys / pcnt : 0;
       const double vz = pcnt > 0 ? vzs / pcnt : 0;
-      phi_w1[i] = (float)p;
-      phi_w2[i] = (float)p;
+      const float pf = (float)p;
+      phi_w1[i] = pf;
+      phi_w2[i] = pf;
       rho[i] = (float)r;
       ux[i] = (float)vx;
       uy[i] = (float)vy;
@@ -301,12 +306,12 @@ __device__ void fresh_block(Geom g, double rhoG, double rhoL, const int32_t *__r
       for (int k = 0; k < Q27; k++) {
         const double eu = EX27[k] * vx + EY27[k] * vy + EZ27[k] * vz;
         const double Qi = eu * (INV_CS2_27 + INV_2CS4_27 * eu) - 1.5 * u2;
-        f[k * g.S + i] = (float)((double)w27(k) * (r + rhoPhase * Qi));
+        C::st(f, k * g.S + i, (float)((double)w27(k) * (r + rhoPhase * Qi)), w27(k));
       }
 #pragma unroll
       for (int k = 0; k < Q7; k++) {
         const double eu = EX7[k] * vx + EY7[k] * vy + EZ7[k] * vz;
-        h[k * g.S + i] = (float)((double)w7(k) * p * (1 + INV_CS2_7 * eu));
+        C::st(h, k * g.S + i, (float)((double)w7(k) * p * (1 + INV_CS2_7 * eu)), w7(k) * pf);
       }
     }
     __threadfence();
@@ -323,25 +328,8 @@ __global__ __launch_bounds__(1024) void k_fresh(Geom g, double rhoG, double rhoL
                                                 float *f, float *h) {
   int count = ctrl->n_fresh;
   if (count > cap) count = cap;
-  fresh_block(g, rhoG, rhoL, list, count, level_scratch, tag, phi_fb, phi_w1, phi_w2, rho, ux, uy,
-              uz, f, h);
-}
-
-// Sorts both lists of a retag, then runs handleFreshCells: one launch for the
-// single-workgroup part of the fused step's band update.
-__global__ __launch_bounds__(1024) void k_sort_fresh(Geom g, double rhoG, double rhoL,
-                                                     int32_t *fresh, int32_t *dead, Ctrl *ctrl,
-                                                     int cap, int *level_scratch,
-                                                     const uint8_t *__restrict__ tag,
-                                                     float *phi, float *rho, float *ux, float *uy,
-                                                     float *uz, float *f, float *h) {
-  __shared__ int32_t sh[kSortLds];
-  int nf = ctrl->n_fresh, nd = ctrl->n_dead;
-  if (nf > cap) nf = cap;
-  if (nd > cap) nd = cap;
-  sort_list_block(fresh, nf, sh);
-  sort_list_block(dead, nd, sh);
-  fresh_block(g, rhoG, rhoL, fresh, nf, level_scratch, tag, phi, phi, phi, rho, ux, uy, uz, f, h);
+  fresh_block<ST_F32>(g, rhoG, rhoL, list, count, level_scratch, tag, phi_fb, phi_w1, phi_w2, rho,
+                      ux, uy, uz, f, h);
 }
 
 // computePhiGradients (src/sim.js:176-199): all cells, clamped indices.

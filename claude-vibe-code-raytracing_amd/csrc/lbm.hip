@@ -72,6 +72,13 @@ struct lbm_sim {
   bool need_full_retag = true;  // next coupled step must retag the whole grid
   bool sphere_dev_valid = false;
   std::vector<lbm_link> links;  // kept by lbm_stream_hydro_macro_links
+  // Compressed populations of the fused step (lbm_set_storage): a pair of
+  // 16-bit f and h buffers; cf[ccur]/ch[ccur] hold the reference's f and h when
+  // comp_auth is set, and the float32 buffers are then stale.
+  int storage = LBM_STORAGE_F32;
+  bool comp_auth = false;
+  int ccur = 0;
+  void *cf[2] = {nullptr, nullptr}, *ch[2] = {nullptr, nullptr};
   // Timing.
   Timer timer;
   std::string err;
@@ -139,6 +146,35 @@ static int ensure_scratch(lbm_sim *s, size_t bytes) {
   s->scratch_bytes = bytes;
   return LBM_OK;
 }
+
+// Makes the float32 populations current again after fused steps with
+// compressed storage: decodes the reference's f and h (h against the current
+// phi) into fbuf[fcur] and hbuf[hcur], on stream st.
+static int ensure_f32(lbm_sim *s, hipStream_t st) {
+  if (!s->comp_auth) return LBM_OK;
+  const int64_t n = s->g.n, S = s->g.S;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  if (s->storage == LBM_STORAGE_F16)
+    k_decode<ST_F16><<<blocks, 256, 0, st>>>(n, S, (const _Float16 *)s->cf[s->ccur],
+                                             (const _Float16 *)s->ch[s->ccur], PHI(s), F(s), H(s));
+  else
+    k_decode<ST_BF16><<<blocks, 256, 0, st>>>(n, S, (const uint16_t *)s->cf[s->ccur],
+                                              (const uint16_t *)s->ch[s->ccur], PHI(s), F(s),
+                                              H(s));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(s, LBM_ERR_HIP, std::string("k_decode: ") + hipGetErrorString(e));
+  s->comp_auth = false;
+  return LBM_OK;
+}
+
+// CHECK_SIM for entry points that read or write the float32 populations or
+// anything their compressed baseline depends on.
+#define CHECK_SIM_F32(sim)                                  \
+  do {                                                      \
+    if (!(sim)) return LBM_ERR_ARG;                         \
+    const int rc_f32_ = ensure_f32((sim), (sim)->stream);   \
+    if (rc_f32_) return rc_f32_;                            \
+  } while (0)
 
 static int ensure_links(lbm_sim *s, int64_t cap) {
   if (cap > INT32_MAX) return fail(s, LBM_ERR_CAPACITY, "link list capacity exceeds 2^31");
@@ -318,7 +354,8 @@ extern "C" void lbm_destroy(lbm_sim *s) {
                   s->rho, s->ux, s->uy, s->uz, s->tag, s->tagPrev, s->usx, s->usy, s->usz, s->gx,
                   s->gy, s->gz, s->lap, s->bfx, s->bfy, s->bfz, s->fresh, s->dead,
                   s->level_scratch, s->ctrl, s->link_key, s->link_m, s->imp, s->scratch,
-                  s->mask_dev, s->mus_dev, s->partials};
+                  s->mask_dev, s->mus_dev, s->partials, s->cf[0], s->cf[1], s->ch[0],
+                  s->ch[1]};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (s->ctrl_host) hipHostFree(s->ctrl_host);
@@ -397,18 +434,18 @@ static int init_columns(lbm_sim *s, const double *yw, double waterline) {
 }
 
 extern "C" int lbm_init_flat(lbm_sim *s, double waterline_y) {
-  CHECK_SIM(s);
+  CHECK_SIM_F32(s);
   return init_columns(s, nullptr, waterline_y);
 }
 
 extern "C" int lbm_init_waterline_map(lbm_sim *s, const double *yw) {
-  CHECK_SIM(s);
+  CHECK_SIM_F32(s);
   if (!yw) return fail(s, LBM_ERR_ARG, "yw is NULL");
   return init_columns(s, yw, 0.0);
 }
 
 extern "C" int lbm_seed_hydrostatic(lbm_sim *s) {
-  CHECK_SIM(s);
+  CHECK_SIM_F32(s);
   const lbm_params &p = s->params;
   dim3 grid((s->nx + 63) / 64, s->nz);
   k_seed_hydrostatic<<<grid, 64, 0, s->stream>>>(s->g, p.rhoG, p.rhoL - p.rhoG, p.gravity,
@@ -437,7 +474,7 @@ static int upload_sphere(lbm_sim *s, const lbm_sphere *b) {
 }
 
 extern "C" int lbm_retag_cells(lbm_sim *s, const lbm_sphere *solid) {
-  CHECK_SIM(s);
+  CHECK_SIM_F32(s);
   int rc;
   if (solid && (rc = upload_sphere(s, solid))) return rc;
   rc = retag_full(s, solid ? 1 : 0);
@@ -447,7 +484,7 @@ extern "C" int lbm_retag_cells(lbm_sim *s, const lbm_sphere *solid) {
 }
 
 extern "C" int lbm_retag_cells_mask(lbm_sim *s, const uint8_t *mask, const float *us) {
-  CHECK_SIM(s);
+  CHECK_SIM_F32(s);
   if (!mask || !us) return fail(s, LBM_ERR_ARG, "mask/us is NULL");
   const int64_t n = s->g.n;
   if (!s->mask_dev) {
@@ -471,12 +508,12 @@ static int fresh_pass(lbm_sim *s, float *phi_fb, float *phi_w1, float *phi_w2, f
 }
 
 extern "C" int lbm_handle_fresh_cells(lbm_sim *s) {
-  CHECK_SIM(s);
+  CHECK_SIM_F32(s);
   return fresh_pass(s, PHIPREV(s), PHI(s), PHIPREV(s), F(s), H(s));
 }
 
 extern "C" int lbm_compute_phi_gradients(lbm_sim *s) {
-  CHECK_SIM(s);
+  CHECK_SIM_F32(s);
   k_gradients<<<cell_grid(s), kCellBlock, 0, s->stream>>>(s->g, PHI(s), s->gx, s->gy, s->gz,
                                                           s->lap);
   s->grads_stale = false;
@@ -484,7 +521,7 @@ extern "C" int lbm_compute_phi_gradients(lbm_sim *s) {
 }
 
 extern "C" int lbm_collide_phase(lbm_sim *s) {
-  CHECK_SIM(s);
+  CHECK_SIM_F32(s);
   const Derived d = derive(s->params);
   k_collide_phase<<<cell_grid(s), kCellBlock, 0, s->stream>>>(s->g, d, H(s), H2(s), PHI(s), s->ux,
                                                               s->uy, s->uz, s->tag, s->gx, s->gy,
@@ -493,13 +530,13 @@ extern "C" int lbm_collide_phase(lbm_sim *s) {
 }
 
 extern "C" int lbm_stream_phase(lbm_sim *s) {
-  CHECK_SIM(s);
+  CHECK_SIM_F32(s);
   k_stream_phase<<<cell_grid(s), kCellBlock, 0, s->stream>>>(s->g, H2(s), H(s), s->tag);
   return check_launch(s, "k_stream_phase");
 }
 
 extern "C" int lbm_collide_hydro(lbm_sim *s) {
-  CHECK_SIM(s);
+  CHECK_SIM_F32(s);
   const Derived d = derive(s->params);
   k_collide_hydro<<<cell_grid(s), kCellBlock, 0, s->stream>>>(
       s->g, d, F(s), F2(s), s->rho, s->ux, s->uy, s->uz, PHI(s), s->tag, s->gx, s->gy, s->gz,
@@ -583,7 +620,7 @@ static void link_to_record(const lbm_sim *s, int64_t key, double m, lbm_link *L)
 }
 
 extern "C" int lbm_stream_hydro_macro(lbm_sim *s, lbm_sphere *solid) {
-  CHECK_SIM(s);
+  CHECK_SIM_F32(s);
   std::vector<int64_t> keys;
   std::vector<double> ms;
   int rc = stream_macro_pass(s, solid != nullptr, &keys, &ms);
@@ -599,7 +636,7 @@ extern "C" int lbm_stream_hydro_macro(lbm_sim *s, lbm_sphere *solid) {
 }
 
 extern "C" int lbm_stream_hydro_macro_links(lbm_sim *s, int *count) {
-  CHECK_SIM(s);
+  CHECK_SIM_F32(s);
   std::vector<int64_t> keys;
   std::vector<double> ms;
   int rc = stream_macro_pass(s, true, &keys, &ms);
@@ -619,7 +656,7 @@ extern "C" int lbm_get_links(lbm_sim *s, lbm_link *out, int cap) {
 }
 
 extern "C" int lbm_sim_step(lbm_sim *s, lbm_sphere *solid) {
-  CHECK_SIM(s);
+  CHECK_SIM_F32(s);
   int rc;
   if ((rc = lbm_retag_cells(s, solid))) return rc;
   if ((rc = lbm_handle_fresh_cells(s))) return rc;
@@ -704,6 +741,11 @@ static void *field_ptr(lbm_sim *s, int field, int *q, size_t *elem) {
 
 extern "C" int lbm_download(lbm_sim *s, int field, void *host) {
   CHECK_SIM(s);
+  if (field == LBM_FIELD_F || field == LBM_FIELD_F2 || field == LBM_FIELD_H ||
+      field == LBM_FIELD_H2) {
+    const int rc = ensure_f32(s, s->stream);
+    if (rc) return rc;
+  }
   if (!host) return fail(s, LBM_ERR_ARG, "host pointer is NULL");
   if (field >= LBM_FIELD_GX && field <= LBM_FIELD_LAP && s->grads_stale) {
     // After a fused step the gradients of the last step are those of phiPrev
@@ -734,7 +776,7 @@ extern "C" int lbm_download(lbm_sim *s, int field, void *host) {
 }
 
 extern "C" int lbm_upload(lbm_sim *s, int field, const void *host) {
-  CHECK_SIM(s);
+  CHECK_SIM_F32(s);
   if (!host) return fail(s, LBM_ERR_ARG, "host pointer is NULL");
   int q;
   size_t elem;
@@ -759,6 +801,49 @@ extern "C" int lbm_upload(lbm_sim *s, int field, const void *host) {
     s->need_full_retag = true;
   if (field >= LBM_FIELD_GX && field <= LBM_FIELD_LAP) s->grads_stale = false;
   return sync(s);
+}
+
+extern "C" int lbm_set_storage(lbm_sim *s, int kind) {
+  CHECK_SIM(s);
+  if (kind != LBM_STORAGE_F32 && kind != LBM_STORAGE_F16 && kind != LBM_STORAGE_BF16)
+    return fail(s, LBM_ERR_ARG, "unknown storage kind");
+  if (kind == s->storage) return LBM_OK;
+  int rc = ensure_f32(s, s->stream);
+  if (rc) return rc;
+  if (kind != LBM_STORAGE_F32 && !s->cf[0]) {
+    const int64_t S = s->g.S;
+    for (int b = 0; b < 2; b++) {
+      HIP_TRY(s, hipMalloc(&s->cf[b], S * 27 * 2));
+      HIP_TRY(s, hipMalloc(&s->ch[b], S * 7 * 2));
+    }
+  }
+  s->storage = kind;
+  return sync(s);
+}
+
+extern "C" int lbm_get_storage(const lbm_sim *s) { return s ? s->storage : LBM_ERR_ARG; }
+
+extern "C" int lbm_mass(lbm_sim *s, double *phi_sum, double *rho_sum) {
+  CHECK_SIM(s);
+  const int64_t n = s->g.n;
+  const int64_t blocks = (n + 255) / 256;
+  int rc = ensure_scratch(s, blocks * 2 * sizeof(double));
+  if (rc) return rc;
+  k_mass_partials<<<(unsigned)blocks, 256, 0, s->stream>>>(n, s->g, s->tag, PHI(s), s->rho,
+                                                           (double *)s->scratch);
+  if ((rc = check_launch(s, "k_mass_partials"))) return rc;
+  std::vector<double> part(2 * blocks);
+  HIP_TRY(s, hipMemcpyAsync(part.data(), s->scratch, 2 * blocks * sizeof(double),
+                            hipMemcpyDeviceToHost, s->stream));
+  if ((rc = sync(s))) return rc;
+  double a = 0, b = 0;
+  for (int64_t k = 0; k < blocks; k++) {
+    a += part[2 * k];
+    b += part[2 * k + 1];
+  }
+  if (phi_sum) *phi_sum = a;
+  if (rho_sum) *rho_sum = b;
+  return LBM_OK;
 }
 
 extern "C" int lbm_pack_volume(lbm_sim *s, uint8_t *out) {

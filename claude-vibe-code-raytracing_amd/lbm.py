@@ -57,6 +57,9 @@ class Link(C.Structure):
 _lib = None
 
 
+STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
+
+
 def lib():
     """Loads liblbm; raises if it has not been built."""
     global _lib
@@ -94,6 +97,9 @@ def lib():
         "lbm_download": (C.c_int, [P, C.c_int, P]),
         "lbm_upload": (C.c_int, [P, C.c_int, P]),
         "lbm_pack_volume": (C.c_int, [P, C.POINTER(C.c_uint8)]),
+        "lbm_set_storage": (C.c_int, [P, C.c_int]),
+        "lbm_get_storage": (C.c_int, [P]),
+        "lbm_mass": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "lbm_fresh_cells": (C.c_int, [P, C.POINTER(C.c_int32), C.c_int, C.POINTER(C.c_int)]),
         "lbm_dead_cells": (C.c_int, [P, C.POINTER(C.c_int32), C.c_int, C.POINTER(C.c_int)]),
         "lbm_sphere_init": (None, [SP, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -302,6 +308,22 @@ class Sim:
         if name in FIELD_IDS:
             return self.get(name)
         raise AttributeError(name)
+
+    def set_storage(self, kind):
+        """Population storage of the fused step: 'f32' (reference-exact), 'f16' or
+        'bf16' (16-bit deviations, FP32 collide arithmetic)."""
+        self._ck(self.L.lbm_set_storage(self.h, STORAGE[kind]), "lbm_set_storage")
+
+    @property
+    def storage(self):
+        k = self.L.lbm_get_storage(self.h)
+        return {v: n for n, v in STORAGE.items()}[k]
+
+    def mass(self):
+        """FP64 sums of phi and of rho over fluid cells."""
+        a, b = C.c_double(), C.c_double()
+        self._ck(self.L.lbm_mass(self.h, C.byref(a), C.byref(b)), "lbm_mass")
+        return a.value, b.value
 
     def pack(self):
         out = np.empty(2 * self.n, np.uint8)

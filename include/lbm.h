@@ -180,12 +180,32 @@ int lbm_absorb_dead_cells(lbm_sim *sim, lbm_sphere *solid);
  * per-pass path in the last bits.  solid may be NULL (no body). */
 int lbm_step(lbm_sim *sim, lbm_sphere *solid, int nsteps);
 
+/* ------------------------------------------------------------------ storage */
+/* Population storage of the fused step.  No reference counterpart (the
+ * reference stores float32, src/sim.js:35-38).  LBM_STORAGE_F32 is the
+ * reference-exact form.  LBM_STORAGE_F16 / LBM_STORAGE_BF16 store every
+ * population as a 16-bit deviation from its direction's baseline (f_k - w_k,
+ * h_k - w7_k phi) and compute the collide in FP32; the momentum-exchange
+ * force/torque sums stay FP64.  Results then differ from the reference by the
+ * storage rounding; the other fields (phi, rho, u, tags) stay float32.  The
+ * float32 populations are decoded on demand (field reads and writes, per-pass
+ * calls); after a compressed lbm_step only f and h are defined, not f2/h2. */
+#define LBM_STORAGE_F32 0
+#define LBM_STORAGE_F16 1
+#define LBM_STORAGE_BF16 2
+int lbm_set_storage(lbm_sim *sim, int kind);
+int lbm_get_storage(const lbm_sim *sim);
+/* Global mass check: FP64 sums of phi and of rho (sum f) over fluid cells,
+ * added in a fixed order.  No reference counterpart. */
+int lbm_mass(lbm_sim *sim, double *phi_sum, double *rho_sum);
+
 /* ------------------------------------------------------------------ batches */
 /* Many simulations stepped together.  No reference counterpart: the reference
  * runs one Sim per Web Worker (src/worker.js:11-13); a batch steps any number
  * of independent Sims (each with its own size, parameters and sphere) with
  * launches that cover all of them, so small scenes fill the GPU together.
- * Every member's results are those of lbm_step on it alone, bit for bit. */
+ * Every member's results are those of lbm_step on it alone, bit for bit.
+ * Members of different storage kinds step in separate launches. */
 typedef struct lbm_batch lbm_batch;
 int lbm_batch_create(lbm_batch **out);
 void lbm_batch_destroy(lbm_batch *batch);

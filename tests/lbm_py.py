@@ -8,5 +8,6 @@ if PKG not in sys.path:
     sys.path.insert(0, PKG)
 
 import lbm  # noqa: E402
+import scenegen  # noqa: E402
 
 HEADER = os.path.join(ROOT, "include", "lbm.h")
