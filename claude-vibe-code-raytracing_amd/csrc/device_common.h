@@ -84,12 +84,21 @@ __device__ __forceinline__ void cell_xyz(int nx, int nxny, int64_t i, int &x, in
   x = r - y * nx;
 }
 
-// Tiles from the one holding the band box's first cell to the one holding its
-// last cell: every cell of the box lies in one of them.
-__host__ __device__ inline void box_tile_range(const TileMap &m, const int box[6], int &lo,
-                                               int &hi) {
-  lo = (int)(((int64_t)box[4] * m.nxny + (int64_t)box[2] * m.nx + box[0]) >> kTileShift);
-  hi = (int)(((int64_t)box[5] * m.nxny + (int64_t)box[3] * m.nx + box[1]) >> kTileShift);
+// True when tile t holds a cell of the band box's rows: in some box plane z the
+// tile lies between the tiles of that plane's first and last box cell.  Every
+// cell of the box lies in such a tile.
+__host__ __device__ inline bool tile_in_box_rows(const TileMap &m, const int box[6], int t) {
+  const int64_t c0 = (int64_t)t << kTileShift, c1 = c0 + (1 << kTileShift) - 1;
+  int za = (int)(c0 / m.nxny), zb = (int)(c1 / m.nxny);
+  za = za < box[4] ? box[4] : za;
+  zb = zb > box[5] ? box[5] : zb;
+  for (int z = za; z <= zb; z++) {
+    const int64_t p = (int64_t)z * m.nxny;
+    const int lo = (int)((p + (int64_t)box[2] * m.nx + box[0]) >> kTileShift);
+    const int hi = (int)((p + (int64_t)box[3] * m.nx + box[1]) >> kTileShift);
+    if (t >= lo && t <= hi) return true;
+  }
+  return false;
 }
 
 __device__ __forceinline__ int64_t cell_index(const Geom &g, int x, int y, int z) {

@@ -189,17 +189,63 @@ __device__ __forceinline__ void collide_hydro_cell(const Derived &d, R sumF, R p
   HydroUnroll<Q27, R>::run(d, sumF, rhoPhase, negU2half, uF, uxe, uye, uze, fx, fy, fz, fin, fout);
 }
 
+// Direction masks: bit k set when the component of e_k along the axis is
+// positive (P) or negative (M).
+template <int A, int SIGN>
+struct DirMask {
+  static constexpr uint32_t make() {
+    uint32_t m = 0;
+    for (int k = 0; k < Q27; k++) {
+      const int e = A == 0 ? EX27[k] : A == 1 ? EY27[k] : EZ27[k];
+      if (e == SIGN) m |= 1u << k;
+    }
+    return m;
+  }
+  static constexpr uint32_t v = make();
+};
+
+// Source mask of interior cell (x, y, z) when no solid cell is within reach:
+// walls are exactly the grid faces (src/sim.js:226-229), so source x - e_x is a
+// wall when x = 1 and e_x = +1 or x = nx - 2 and e_x = -1, and likewise for y
+// and z.  Bit k is set when the source of direction k is not fluid.
+__device__ __forceinline__ uint32_t wall_mask(const Geom &g, int x, int y, int z) {
+  uint32_t m = 0;
+  if (x == 1) m |= DirMask<0, 1>::v;
+  if (x == g.nx - 2) m |= DirMask<0, -1>::v;
+  if (y == 1) m |= DirMask<1, 1>::v;
+  if (y == g.ny - 2) m |= DirMask<1, -1>::v;
+  if (z == 1) m |= DirMask<2, 1>::v;
+  if (z == g.nz - 2) m |= DirMask<2, -1>::v;
+  return m;
+}
+
+// Source masks from the neighbours' tags (bit k: source of direction k not
+// fluid / solid).
+__device__ __forceinline__ void tag_masks(const Geom &g, int64_t i,
+                                          const uint8_t *__restrict__ tag, uint32_t &nonfluid,
+                                          uint32_t &solid) {
+  nonfluid = 0;
+  solid = 0;
+#pragma unroll
+  for (int k = 1; k < Q27; k++) {
+    const int64_t s = i - ((int64_t)EZ27[k] * g.nxny + (int64_t)EY27[k] * g.nx + EX27[k]);
+    const uint8_t st = tag[s];
+    nonfluid |= (uint32_t)(st != TAG_FLUID) << k;
+    solid |= (uint32_t)(st == TAG_SOLID) << k;
+  }
+}
+
 // Pull stream of the 27 populations with halfway bounce-back
-// (src/sim.js:509-543).  All source tags are read first and every load address
-// is chosen without branching (own opposite population at a non-fluid source),
-// so the 27 loads are in flight together; the moving-wall term and the momentum
+// (src/sim.js:509-543), given the source masks.  Every load address is chosen
+// without branching (own opposite population at a non-fluid source), so the 27
+// loads are in flight together; the moving-wall term and the momentum
 // exchange run only for solid links.  fk stays the unrounded value the
 // reference sums and exchanges; fp receives it rounded to float.  The momentum
 // exchange accumulates in FP64 whatever the arithmetic type.
 template <int KIND>
 __device__ __forceinline__ void pull27(const Geom &g, int64_t i,
                                        const typename Codec<KIND>::T *__restrict__ f2,
-                                       const uint8_t *__restrict__ tag,
+                                       uint32_t nonfluid, uint32_t solid,
                                        const float *__restrict__ usx,
                                        const float *__restrict__ usy,
                                        const float *__restrict__ usz,
@@ -208,18 +254,9 @@ __device__ __forceinline__ void pull27(const Geom &g, int64_t i,
                                        double mom[6], float fp[Q27],
                                        typename Codec<KIND>::R &pSumF,
                                        typename Codec<KIND>::R &mx, typename Codec<KIND>::R &my,
-                                       typename Codec<KIND>::R &mz, uint32_t &nonfluid) {
+                                       typename Codec<KIND>::R &mz) {
   using C = Codec<KIND>;
   using R = typename C::R;
-  nonfluid = 0;
-  uint32_t solid = 0;
-#pragma unroll
-  for (int k = 1; k < Q27; k++) {
-    const int64_t s = i - ((int64_t)EZ27[k] * g.nxny + (int64_t)EY27[k] * g.nx + EX27[k]);
-    const uint8_t st = tag[s];
-    nonfluid |= (uint32_t)(st != TAG_FLUID) << k;
-    solid |= (uint32_t)(st == TAG_SOLID) << k;
-  }
   float raw[Q27];
 #pragma unroll
   for (int k = 0; k < Q27; k++) {
@@ -525,15 +562,25 @@ __device__ __forceinline__ void phase_stream_body(const Scene &sc, int lb, int q
   const float *phi = par(sc.phi, q);
   float *phi_next = par(sc.phi, q ^ 1);
   const uint8_t *tag = sc.tag;
-  if (tag[i] != TAG_FLUID || !interior(g, x, y, z)) {
+  // Outside the band box the only non-fluid cells are the walls on the grid
+  // faces (solid cells lie inside it with a margin of 2).
+  int box[6];
+#pragma unroll
+  for (int j = 0; j < 6; j++) box[j] = sc.ctrl->box_fused[j];
+  const bool near = in_box(box, x, y, z);
+  if (!interior(g, x, y, z) || (near && tag[i] != TAG_FLUID)) {
     phi_next[i] = phi[i];
     return;
   }
   uint32_t nonfluid = 0;
+  if (near) {
 #pragma unroll
-  for (int k = 1; k < Q7; k++) {
-    const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
-    nonfluid |= (uint32_t)(tag[s] != TAG_FLUID) << k;
+    for (int k = 1; k < Q7; k++) {
+      const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
+      nonfluid |= (uint32_t)(tag[s] != TAG_FLUID) << k;
+    }
+  } else {
+    nonfluid = wall_mask(g, x, y, z) & 0x7eu;  // D3Q7 directions 1..6
   }
   float hv[Q7];
   pull7<KIND>(g, i, h2, phi, nonfluid, hv);
@@ -589,11 +636,9 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
 #pragma unroll
   for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
   // Link momentum can only arise next to the sphere, inside the band box: the
-  // tiles from the one holding the box's first cell to the one holding its
-  // last store their sums (zero or not), and the body update reads those.
-  int tlo, thi;
-  box_tile_range(sc.tm, box, tlo, thi);
-  const bool exchange = sc.has_solid != 0 && tile >= tlo && tile <= thi;
+  // tiles holding the box's rows store their sums (zero or not), and the body
+  // update reads exactly those.
+  const bool exchange = sc.has_solid != 0 && tile_in_box_rows(sc.tm, box, tile);
   const int use_bf = first && sc.use_bf;
   double cx = 0, cy = 0, cz = 0;
   if (exchange) {
@@ -602,7 +647,10 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
     cz = ctrl->sphere.cz;
   }
   if (in_grid) {
-    const bool fluid = tag[i] == TAG_FLUID && interior(g, x, y, z);
+    // Solid cells lie inside the band box with a margin of 2, so outside it the
+    // only non-fluid cells are the walls on the grid faces.
+    const bool near = in_box(box, x, y, z);
+    const bool fluid = interior(g, x, y, z) && (!near || tag[i] == TAG_FLUID);
     if (!fluid) {
       if (!COLLIDE) phi_next[i] = phi[i];
     } else {
@@ -610,11 +658,16 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
       const R rhoPhaseOld = R(d.rhoG) + phiOld * R(d.dRho);
       float gxo, gyo, gzo, lapo;
       phi_grad<R>(g, phi, i, phiOld, gxo, gyo, gzo, lapo);
+      uint32_t nonfluid, solid;
+      if (near) tag_masks(g, i, tag, nonfluid, solid);
+      else {
+        nonfluid = wall_mask(g, x, y, z);
+        solid = 0;
+      }
       float fp[Q27];
       R pSumF, mx, my, mz;
-      uint32_t nonfluid;
-      pull27<KIND>(g, i, f2, tag, sc.usx, sc.usy, sc.usz, rhoPhaseOld, exchange, x, y, z, cx, cy,
-                   cz, mom, fp, pSumF, mx, my, mz, nonfluid);
+      pull27<KIND>(g, i, f2, nonfluid, solid, sc.usx, sc.usy, sc.usz, rhoPhaseOld, exchange, x, y,
+                   z, cx, cy, cz, mom, fp, pSumF, mx, my, mz);
       const R rhoN = pSumF > R(0.01) ? pSumF : R(0.01);
       float hp[Q7];
       pull7<KIND>(g, i, h2, phi, nonfluid, hp);
@@ -644,7 +697,7 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
       }
       const float rhof = (float)rhoN, phif = (float)p;
       const float uxf = (float)uxn, uyf = (float)uyn, uzf = (float)uzn;
-      const bool band = in_box(box, x, y, z);
+      const bool band = near;
       // rho and u are read before the end of the call only inside the band (by
       // the fresh-cell pass, the dead-cell impulses and the band collide); the
       // last step of a call stores them everywhere.
@@ -765,9 +818,9 @@ __global__ __launch_bounds__(256) void k_mass_partials(int64_t n, Geom g,
 // ---------------------------------------------------------------- body
 
 // Body update after stream(t) of step q, one 512-thread workgroup per scene:
-//   F/T += link momentum of the tiles from the one holding this step's band
-//          box's first cell to the one holding its last, in ascending tile
-//          order (contiguous chunks per thread, then waves in order);
+//   F/T += link momentum of the tiles holding this step's band box rows, in a
+//          fixed order (planes dealt to waves, tiles to lanes, then lanes and
+//          waves summed in order);
 //   F/T += dead-cell impulses rho_phase(phi) u at the cells that turned solid at
 //          this step's retag (src/solid.js:87-101), in the same chunked order;
 //          a dead cell keeps the phi and u it had when it turned solid, which
@@ -798,24 +851,33 @@ __device__ __forceinline__ void body_body(const Scene &sc, int q, int check_band
   int box[6];
 #pragma unroll
   for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
-  int tlo, thi;
-  box_tile_range(tm, box, tlo, thi);
-  const int ntiles = thi - tlo + 1;
   double acc[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) acc[k] = 0;
   {
-    const int C = (ntiles + kBodyThreads - 1) / kBodyThreads;
-    const int lo = tid * C, hi = min(ntiles, lo + C);
-    for (int c = lo; c < hi; c++) {
-      const int64_t t = tlo + c;
+    // Plane z's tiles run from the one holding its first box cell to the one
+    // holding its last, less those already counted for plane z - 1 (a tile can
+    // span two planes).  Wave w takes planes box[4] + w, + 8, ...; lane l the
+    // tiles lo + l, + 64, ...
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int z = box[4] + wave; z <= box[5]; z += kBodyThreads / 64) {
+      const int64_t p = (int64_t)z * tm.nxny;
+      int lo = (int)((p + (int64_t)box[2] * tm.nx + box[0]) >> kTileShift);
+      const int hi = (int)((p + (int64_t)box[3] * tm.nx + box[1]) >> kTileShift);
+      if (z > box[4]) {
+        const int hprev = (int)((p - tm.nxny + (int64_t)box[3] * tm.nx + box[1]) >> kTileShift);
+        lo = lo > hprev + 1 ? lo : hprev + 1;
+      }
+      for (int t = lo + lane; t <= hi; t += 64) {
 #pragma unroll
-      for (int k = 0; k < 6; k++) acc[k] += sc.partials[t * 6 + k];
+        for (int k = 0; k < 6; k++) acc[k] += sc.partials[(int64_t)t * 6 + k];
+      }
     }
   }
   int nd = ctrl->n_dead;
   if (nd > sc.list_cap) nd = sc.list_cap;
   {
+    // Dead cells: contiguous chunks of the sorted list per thread.
     const int C = (nd + kBodyThreads - 1) / kBodyThreads;
     const int lo = tid * C, hi = min(nd, lo + C);
     for (int c = lo; c < hi; c++) {
