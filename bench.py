@@ -156,6 +156,86 @@ def bench_batch(args, copy_gbps):
     return 0
 
 
+def serve_consume(srv, args, t0, lat, first):
+    """Reads frames until the run's duration ends; returns (stats, seconds)."""
+    last_print = t0
+    while True:
+        now = time.perf_counter()
+        if now - t0 >= args.duration:
+            break
+        fr = srv.next_frame(timeout_ms=100, want_rg8=True, copy=False)
+        if now - last_print > 10:
+            st = srv.stats()
+            print(f"[bench] serve t={now - t0:.0f}s active={st['active']} pending={st['pending']} "
+                  f"frames={st['frames']}", file=sys.stderr, flush=True)
+            last_print = now
+        if fr is None:
+            continue
+        info, _ = fr
+        (first if info.index == 0 else lat).append(info.latency)
+    return srv.stats(), time.perf_counter() - t0
+
+
+def bench_serve(args):
+    """Served scenes (config 4): Poisson arrivals, admission by memory budget,
+    continuous batching, an RG8 frame every 10 steps; aggregate MLUPS and frame
+    latency percentiles over the run."""
+    import threading
+    reqs = scenegen.config4(args.duration, rate=args.rate, seed=args.seed + 3)
+    srv = lbm.Server(int(args.budget_gb * (1 << 30)), max_active=args.max_active)
+    t0 = time.perf_counter()
+    stop = threading.Event()
+    submitted = [0]
+
+    def feeder():
+        for (ta, (nx, ny, nz), params, yw, sph, steps) in reqs:
+            delay = ta - (time.perf_counter() - t0)
+            if delay > 0 and stop.wait(delay):
+                return
+            b = lbm.make_sphere(nx, ny, nz, sph["cx"], sph["cy"], sph["cz"], sph["r"],
+                                sph["density"])
+            try:
+                srv.submit(nx, ny, nz, params, steps, frame_every=10, waterline=yw, sphere=b,
+                           storage=args.storage)
+                submitted[0] += 1
+            except lbm.LbmError:
+                pass
+
+    th = threading.Thread(target=feeder, daemon=True)
+    th.start()
+    lat, first = [], []
+    try:
+        st, dt = serve_consume(srv, args, t0, lat, first)
+    finally:
+        stop.set()
+        th.join()
+        srv.close()
+    frames = len(lat) + len(first)
+    mlups = st["lattice_updates"] / dt / 1e6
+    pct = lambda v, p: round(float(np.percentile(v, p)) * 1e3, 2) if v else None
+    out = {
+        "metric": "served scenes aggregate MLUPS, p99 frame latency (config 4)",
+        "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1, "steps": None,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": ARITH[args.storage], "data": "synthetic",
+        "config": {"workload": "config 4: Poisson arrivals, axes log-uniform 32..192, step budgets "
+                               "500..5000, RG8 frame every 10 steps, admission by memory budget, "
+                               "continuous batching",
+                   "rate_per_s": args.rate, "duration_s": args.duration,
+                   "budget_gib": args.budget_gb, "max_active": args.max_active,
+                   "storage": args.storage, "seed": args.seed + 3},
+        "frame_latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "count": len(lat)},
+        "first_frame_latency_ms": {"p50": pct(first, 50), "p99": pct(first, 99),
+                                   "count": len(first)},
+        "requests": {"offered": len(reqs), "submitted": submitted[0], "admitted": st["admitted"],
+                     "completed": st["completed"], "max_active": st["max_active_seen"]},
+        "gpu_busy_frac": round(st["busy_seconds"] / dt, 4),
+        "frames": frames,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,9 +247,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="only run the plain copy kernel (counter calibration)")
-    ap.add_argument("--workload", choices=["single", "batch"], default="single",
+    ap.add_argument("--workload", choices=["single", "batch", "serve"], default="single",
                     help="single: the 256^3 scene (headline); batch: 64 seeded scenes of "
-                         "32..96 cells per axis stepped together")
+                         "32..96 cells per axis stepped together; serve: the request loop "
+                         "under Poisson arrivals")
+    ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds")
+    ap.add_argument("--rate", type=float, default=20.0, help="serve: arrivals per second")
+    ap.add_argument("--budget-gb", type=float, default=160.0, help="serve: GiB of scene state")
+    ap.add_argument("--max-active", type=int, default=48, help="serve: active scene cap")
     ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--storage", choices=["f32", "f16", "bf16"], default="f32",
                     help="population storage: f32 (reference-exact, FP64 arithmetic) or a "
@@ -185,6 +270,8 @@ def main():
         return 0
     if args.workload == "batch":
         return bench_batch(args, copy_gbps)
+    if args.workload == "serve":
+        return bench_serve(args)
     n = args.size
     sim, b, params, yw, sph = build_scene(n, args.seed)
     sim.set_storage(args.storage)

@@ -40,6 +40,8 @@ struct Codec<ST_F32> {
   static constexpr int bytes = 4;
   __device__ __forceinline__ static float ld(const T *p, int64_t a, float) { return p[a]; }
   __device__ __forceinline__ static void st(T *p, int64_t a, float v, float) { p[a] = v; }
+  // The value st then ld give back.
+  __device__ __forceinline__ static float rt(float v, float) { return v; }
 };
 
 template <>
@@ -53,6 +55,9 @@ struct Codec<ST_F16> {
   __device__ __forceinline__ static void st(T *p, int64_t a, float v, float base) {
     p[a] = (T)(v - base);
   }
+  __device__ __forceinline__ static float rt(float v, float base) {
+    return (float)(T)(v - base) + base;
+  }
 };
 
 template <>
@@ -65,6 +70,9 @@ struct Codec<ST_BF16> {
   }
   __device__ __forceinline__ static void st(T *p, int64_t a, float v, float base) {
     p[a] = f32_to_bf16(v - base);
+  }
+  __device__ __forceinline__ static float rt(float v, float base) {
+    return bf16_to_f32(f32_to_bf16(v - base)) + base;
   }
 };
 

@@ -712,7 +712,16 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
         if (!band) {
           store_stream = false;
           // collide(t+1) with the new macroscopic values (rounded to float as
-          // the reference stores them) and the gradients of phi(t+1).
+          // the reference stores them) and the gradients of phi(t+1).  The
+          // post-stream populations enter as storing and reloading them would
+          // give them back (exact for float32), so a cell collides the same
+          // whether or not a call ends between the stream and the collide.
+          if (KIND != ST_F32) {
+#pragma unroll
+            for (int k = 0; k < Q27; k++) fp[k] = C::rt(fp[k], w27(k));
+#pragma unroll
+            for (int k = 0; k < Q7; k++) hp[k] = C::rt(hp[k], w7(k) * phif);
+          }
           float gx1, gy1, gz1, lap1;
           phi_grad<R>(g, phi_next, i, (R)phif, gx1, gy1, gz1, lap1);
           float hc[Q7];

@@ -1016,3 +1016,7 @@ done:
 // ------------------------------------------------------------------ coupled step
 
 #include "coupled.inc"
+
+// ------------------------------------------------------------------ serving
+
+#include "serve.inc"

@@ -4,8 +4,10 @@ Host-side plumbing for Python callers (tests, the benchmark): it loads
 lib/liblbm.so and exposes the C entry points with the same method names as the
 reference's Sim / RigidSphere (src/sim.js, src/solid.js).  All computation
 happens in the HIP library; this module only moves arguments."""
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -52,6 +54,35 @@ class Params(C.Structure):
 class Link(C.Structure):
     _fields_ = [("x", C.c_int32), ("y", C.c_int32), ("z", C.c_int32), ("k", C.c_int32),
                 ("px", C.c_double), ("py", C.c_double), ("pz", C.c_double)]
+
+
+class Request(C.Structure):
+    """struct lbm_request."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("params", Params),
+                ("waterline", C.POINTER(C.c_double)), ("waterline_y", C.c_double),
+                ("sphere", Sphere), ("has_sphere", C.c_int), ("steps", C.c_int),
+                ("frame_every", C.c_int), ("storage", C.c_int)]
+
+
+class FrameInfo(C.Structure):
+    """struct lbm_frame_info."""
+    _fields_ = [("request", C.c_int64), ("status", C.c_int), ("index", C.c_int),
+                ("step", C.c_int), ("last", C.c_int), ("nx", C.c_int), ("ny", C.c_int),
+                ("nz", C.c_int), ("bytes", C.c_int64), ("t_submit", C.c_double),
+                ("t_ready", C.c_double), ("latency", C.c_double), ("sphere", Sphere)]
+
+
+class ServerStats(C.Structure):
+    """struct lbm_server_stats."""
+    _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
+                ("rejected", C.c_int64), ("active", C.c_int), ("pending", C.c_int),
+                ("max_active_seen", C.c_int), ("queued_frames", C.c_int),
+                ("frames", C.c_int64), ("ticks", C.c_int64), ("lattice_updates", C.c_int64),
+                ("busy_seconds", C.c_double), ("uptime_seconds", C.c_double),
+                ("bytes_in_use", C.c_int64), ("budget", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _lib = None
@@ -123,6 +154,16 @@ def lib():
                                       C.POINTER(C.c_int64)]),
         "lbm_version": (C.c_char_p, []),
         "lbm_probe_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
+        "lbm_state_bytes": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int]),
+        "lbm_server_create": (C.c_int, [C.c_int64, C.c_int, C.c_int64, C.POINTER(P)]),
+        "lbm_server_destroy": (None, [P]),
+        "lbm_server_last_error": (C.c_char_p, [P]),
+        "lbm_server_submit": (C.c_int, [P, C.POINTER(Request), C.POINTER(C.c_int64)]),
+        "lbm_server_next_frame": (C.c_int, [P, C.c_int, C.POINTER(FrameInfo),
+                                            C.POINTER(C.c_uint8), C.c_int64]),
+        "lbm_server_stats_get": (C.c_int, [P, C.POINTER(ServerStats)]),
+        "lbm_server_clock": (C.c_double, [P]),
+        "lbm_server_pause": (C.c_int, [P, C.c_int]),
         "lbm_batch_create": (C.c_int, [C.POINTER(P)]),
         "lbm_batch_destroy": (None, [P]),
         "lbm_batch_last_error": (C.c_char_p, [P]),
@@ -414,3 +455,108 @@ class Batch:
         self._ck(self.L.lbm_batch_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(cnt)),
                  "lbm_batch_kernel_time")
         return ms.value, cnt.value
+
+
+_live_servers = weakref.WeakSet()
+
+
+@atexit.register
+def _close_servers():
+    # The loop thread must stop before the process tears the HIP runtime down.
+    for srv in list(_live_servers):
+        srv.close()
+
+
+class Server:
+    """The serving loop (struct lbm_server): submit requests, read RG8 frames."""
+
+    def __init__(self, hbm_budget, max_active=256, frame_queue_bytes=0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.lbm_server_create(int(hbm_budget), int(max_active), int(frame_queue_bytes),
+                                 C.byref(h))
+        if rc != 0:
+            raise LbmError(f"lbm_server_create = {rc}: {L.lbm_last_error(None).decode()}")
+        self.h = h
+        self.L = L
+        self._buf = np.empty(0, np.uint8)
+        _live_servers.add(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lbm_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc, what):
+        if rc < 0:
+            raise LbmError(f"{what} = {rc}: {self.L.lbm_server_last_error(self.h).decode()}")
+        return rc
+
+    def submit(self, nx, ny, nz, params, steps, frame_every=10, waterline=None, waterline_y=0.0,
+               sphere=None, storage="f32"):
+        """Queues a scene; waterline is an [nz, nx] array of interface heights
+        (lbm_init_waterline_map) or None for a flat interface at waterline_y."""
+        r = Request()
+        r.nx, r.ny, r.nz = nx, ny, nz
+        for k in PARAMS:
+            setattr(r.params, k, float(params.get(k, 0.02 if k == "mobility" else 0.0)))
+        keep = None
+        if waterline is not None:
+            keep = np.ascontiguousarray(waterline, dtype=np.float64).ravel()
+            assert keep.size == nx * nz
+            r.waterline = keep.ctypes.data_as(C.POINTER(C.c_double))
+        r.waterline_y = float(waterline_y)
+        if sphere is not None:
+            C.memmove(C.byref(r.sphere), C.byref(sphere), C.sizeof(Sphere))
+            r.has_sphere = 1
+        r.steps, r.frame_every, r.storage = int(steps), int(frame_every), STORAGE[storage]
+        if self._buf.size < 2 * nx * ny * nz:
+            self._buf = np.empty(2 * nx * ny * nz, np.uint8)
+        rid = C.c_int64()
+        self._ck(self.L.lbm_server_submit(self.h, C.byref(r), C.byref(rid)), "lbm_server_submit")
+        return rid.value
+
+    def next_frame(self, timeout_ms=1000, want_rg8=True, copy=True):
+        """(info, rg8 array or None), or None on timeout.  copy=False returns a
+        view of the server object's frame buffer, valid until the next call."""
+        info = FrameInfo()
+        while True:
+            if want_rg8:
+                buf = self._buf
+                ptr = buf.ctypes.data_as(C.POINTER(C.c_uint8))
+                n = buf.size
+            else:
+                buf, ptr, n = None, None, 0
+            rc = self.L.lbm_server_next_frame(self.h, int(timeout_ms), C.byref(info), ptr, n)
+            if rc == -5 and want_rg8 and info.bytes > n:  # LBM_ERR_CAPACITY: grow, retry
+                self._buf = np.empty(int(info.bytes), np.uint8)
+                continue
+            self._ck(rc, "lbm_server_next_frame")
+            break
+        if rc == 0:
+            return None
+        if not want_rg8:
+            return info, None
+        return info, (buf[:info.bytes].copy() if copy else buf[:info.bytes])
+
+    def stats(self):
+        st = ServerStats()
+        self._ck(self.L.lbm_server_stats_get(self.h, C.byref(st)), "lbm_server_stats_get")
+        return st.as_dict()
+
+    def clock(self):
+        return self.L.lbm_server_clock(self.h)
+
+    def pause(self, paused=True):
+        self._ck(self.L.lbm_server_pause(self.h, 1 if paused else 0), "lbm_server_pause")
+
+
+def state_bytes(nx, ny, nz, storage="f32"):
+    return lib().lbm_state_bytes(nx, ny, nz, STORAGE[storage])
+

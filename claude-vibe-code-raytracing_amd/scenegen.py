@@ -92,3 +92,33 @@ def config2(count=64, seed=2):
         out.append(((nx, ny, nz), dict(PARAMS), yw,
                     dict(cx=cx, cy=cy, cz=cz, r=r, density=density, vx=0.0, vy=0.0, vz=0.0)))
     return out
+
+
+def config4(duration, rate=20.0, seed=4):
+    """Served requests: Poisson arrivals at `rate` per second over `duration`
+    seconds; each axis log-uniform in 32..192 cells; step budget uniform in
+    500..5000; fill, sphere and parameters drawn as in config2.  Returns a list
+    of (t_arrival, dims, params, yw[nz, nx], sphere dict, steps)."""
+    rng = SplitMix64(seed)
+    out = []
+    t = 0.0
+    lo, hi = math.log(32.0), math.log(192.0)
+    while True:
+        t += -math.log(max(1e-300, 1.0 - rng.uniform(0.0, 1.0))) / rate
+        if t >= duration:
+            return out
+        nx, ny, nz = (int(round(math.exp(rng.uniform(lo, hi)))) for _ in range(3))
+        steps = rng.randint(500, 5000)
+        fill = rng.uniform(0.3, 0.6)
+        yw, _ = waterline_modes(rng, nx, nz, (1.0 - fill) * ny)
+        r = min(rng.uniform(3.0, 10.0), 0.2 * min(nx, ny, nz))
+        density = rng.uniform(1.5, 4.0)
+        m = r + 1.5
+        top = float(yw.min()) - r - 1.0
+        lo_y, hi_y = m, min(top, ny - 1 - m)
+        cy = rng.uniform(lo_y, hi_y) if hi_y > lo_y else lo_y
+        cx = rng.uniform(m, nx - 1 - m)
+        cz = rng.uniform(m, nz - 1 - m)
+        out.append((t, (nx, ny, nz), dict(PARAMS), yw,
+                    dict(cx=cx, cy=cy, cz=cz, r=r, density=density), steps))
+

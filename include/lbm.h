@@ -222,6 +222,78 @@ int lbm_batch_step(lbm_batch *batch, int nsteps);
 int lbm_batch_set_timing(lbm_batch *batch, int on);
 int lbm_batch_kernel_time(lbm_batch *batch, const char *name, double *total_ms, int64_t *launches);
 
+/* ------------------------------------------------------------------ serving */
+/* A request loop that keeps many scenes resident and steps them with
+ * continuous batching.  No reference counterpart as an API: it serves the
+ * frame protocol of src/worker.js:94-165 (an RG8 phi+mask volume, as
+ * Sim.packVolume writes it, plus the sphere state, every few substeps) for
+ * many independent scenes at once.  Requests wait in arrival order until their
+ * state fits the device-memory budget and an active slot is free; every active
+ * scene advances in the same launches; a frame is produced every frame_every
+ * steps and at the end of the step budget. */
+typedef struct {
+  int nx, ny, nz;
+  lbm_params params;
+  const double *waterline; /* nz*nx interface heights for lbm_init_waterline_map, or NULL */
+  double waterline_y;      /* lbm_init_flat height when waterline is NULL */
+  lbm_sphere sphere;       /* initial sphere (lbm_sphere_init) when has_sphere */
+  int has_sphere;
+  int steps;               /* step budget */
+  int frame_every;         /* steps between frames */
+  int storage;             /* LBM_STORAGE_* */
+} lbm_request;
+
+typedef struct {
+  int64_t request;   /* id from lbm_server_submit */
+  int status;        /* LBM_OK, or the error that ended the request */
+  int index;         /* frame number within the request */
+  int step;          /* steps done when the frame was packed */
+  int last;          /* 1 on the request's final frame */
+  int nx, ny, nz;
+  int64_t bytes;     /* 2 * nx * ny * nz */
+  double t_submit;   /* server clock (seconds) at submission */
+  double t_ready;    /* server clock when the frame was ready */
+  double latency;    /* t_ready - previous frame's t_ready (t_submit for the first) */
+  lbm_sphere sphere; /* sphere state at the frame */
+} lbm_frame_info;
+
+typedef struct {
+  int64_t submitted, admitted, completed, rejected;
+  int active, pending, max_active_seen, queued_frames;
+  int64_t frames, ticks;
+  int64_t lattice_updates; /* cell updates done (all cells of every stepped scene) */
+  double busy_seconds;     /* time inside batched steps */
+  double uptime_seconds;
+  int64_t bytes_in_use, budget;
+} lbm_server_stats;
+
+typedef struct lbm_server lbm_server;
+/* Device bytes lbm_create + lbm_set_storage take for a grid (admission). */
+int64_t lbm_state_bytes(int nx, int ny, int nz, int storage);
+/* hbm_budget: device bytes the resident scenes may take; max_active: active
+ * scene cap; frame_queue_bytes: unread frame bytes before the loop waits for
+ * the reader (0: 1 GiB).  Starts the loop thread. */
+int lbm_server_create(int64_t hbm_budget, int max_active, int64_t frame_queue_bytes,
+                      lbm_server **out);
+/* Stops the loop and frees every resident scene. */
+void lbm_server_destroy(lbm_server *server);
+const char *lbm_server_last_error(lbm_server *server);
+/* Queues a request (copied); *id receives its id.  LBM_ERR_CAPACITY when its
+ * state alone exceeds the budget.  Thread-safe. */
+int lbm_server_submit(lbm_server *server, const lbm_request *request, int64_t *id);
+/* Pops the oldest frame, waiting up to timeout_ms: returns 1 with *info and the
+ * 2n RG8 bytes in rg8 (cap bytes available; NULL to drop them), 0 on timeout,
+ * negative on error.  LBM_ERR_CAPACITY when cap is too small: the frame stays
+ * queued and info->bytes gives the size needed.  Thread-safe. */
+int lbm_server_next_frame(lbm_server *server, int timeout_ms, lbm_frame_info *info, uint8_t *rg8,
+                          int64_t cap);
+int lbm_server_stats_get(lbm_server *server, lbm_server_stats *out);
+/* Seconds since the server was created (the clock of the frame times). */
+double lbm_server_clock(lbm_server *server);
+/* paused != 0: the loop admits and steps nothing after the tick in progress
+ * (requests still queue); 0: resume.  Lets a caller queue a burst at once. */
+int lbm_server_pause(lbm_server *server, int paused);
+
 /* ------------------------------------------------------------------ fields */
 
 /* Number of elements of a field (n, n*27 or n*7). */
