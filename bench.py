@@ -9,12 +9,15 @@ absorbDeadCells; sphere step.  Prints one JSON line.
 MLUPS counts all nx*ny*nz cells, walls included.  The roofline object is for
 the dominant kernel (stream + macro + next collide), timed with HIP events on
 the library's stream inside the timed region; its algorithmic traffic is 281 B
-per cell (27+7 populations read and written, phi(t), phi(t+1) and the tag
-read; bulk cells store rho and u only at the last step of a call).  The
-whole-step fraction uses 297 B per cell (the same plus the phi write of the
-phase pass and the rho/u writes).  `traffic` is the measured HBM bytes per
-launch from profiles/stream_collide_256cube_traffic.json (rocprofv3
-FETCH_SIZE/WRITE_SIZE, calibrated on a copy kernel), for the 256^3 scene only.
+per cell in float32 (27+7 populations read and written, phi(t), phi(t+1) and
+the tag read; bulk cells store rho and u only at the last step of a call) and
+145 B with 16-bit populations (--storage f16/bf16).  The whole-step fraction
+uses 297 B (161 B) per cell (the same plus the phi write of the phase pass and
+the rho/u writes).  `traffic` is the measured HBM bytes per launch from the
+committed counter profiles (profiles/stream_collide_256cube*_traffic.json,
+rocprofv3 FETCH_SIZE/WRITE_SIZE calibrated on a copy kernel), for the 256^3
+scene only.  Other workloads: --workload batch (config 2) and --workload serve
+(config 4, Poisson arrivals into the serving loop).
 The CPU baseline is the oracle (a single-threaded C restatement of the
 reference) on the same scene for a few steps."""
 import argparse
@@ -42,16 +45,17 @@ HBM_PEAK_GBS = 8000.0
 # rho/u written + tag.
 KERNEL_BYTES_PER_LU = {"f32": 281, "f16": 145, "bf16": 145}
 STEP_BYTES_PER_LU = {"f32": 297, "f16": 161, "bf16": 161}
-TRAFFIC_JSON = os.path.join(ROOT, "profiles", "stream_collide_256cube_traffic.json")
+TRAFFIC_JSON = {"f32": os.path.join(ROOT, "profiles", "stream_collide_256cube_traffic.json"),
+                "f16": os.path.join(ROOT, "profiles", "stream_collide_256cube_f16_traffic.json")}
 
 
-def measured_traffic(n):
+def measured_traffic(n, storage):
     """HBM bytes per launch of the dominant kernel from the committed counter
-    profile, when it was taken on the same grid."""
+    profile, when it was taken on the same grid and storage."""
     try:
-        with open(TRAFFIC_JSON) as fh:
+        with open(TRAFFIC_JSON[storage]) as fh:
             t = json.load(fh)
-    except (OSError, ValueError):
+    except (KeyError, OSError, ValueError):
         return None
     return t["traffic_bytes"] if t.get("cells") == n ** 3 else None
 STORAGE_NAME = {"f32": "float32", "f16": "fp16-deviation", "bf16": "bf16-deviation"}
@@ -317,7 +321,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": measured_traffic(n) if args.storage == "f32" else None,
+                     "traffic": measured_traffic(n, args.storage),
                      "kernel": "k1_stream_collide", "bytes_per_lu": kb,
                      "avg_ms": round(avg_s * 1e3, 4)},
         "whole_step_hbm_frac": round(STEP_BYTES_PER_LU[args.storage] * mlups * 1e6 /
