@@ -87,3 +87,11 @@ def test_dropin_scenarios():
     r = run_node("scenarios.mjs", timeout=400)
     assert r.returncode == 0, r.stdout[-2000:]
     assert "ALL SCENARIOS OK" in r.stdout
+
+
+def test_frame_delivery_end_to_end():
+    """frame_server.js + remote_worker.js: the worker protocol over a WebSocket,
+    frames equal to packVolume of the same scene stepped locally."""
+    r = run_node("frame_delivery.mjs", timeout=120)
+    assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-2000:])
+    assert "OK frames=" in r.stdout
