@@ -290,8 +290,9 @@ int lbm_server_next_frame(lbm_server *server, int timeout_ms, lbm_frame_info *in
 int lbm_server_stats_get(lbm_server *server, lbm_server_stats *out);
 /* Seconds since the server was created (the clock of the frame times). */
 double lbm_server_clock(lbm_server *server);
-/* paused != 0: the loop admits and steps nothing after the tick in progress
- * (requests still queue); 0: resume.  Lets a caller queue a burst at once. */
+/* paused != 0: no scene joins or steps after the tick in progress (requests
+ * still queue and are prepared); 0: resume.  Lets a caller queue a burst at
+ * once. */
 int lbm_server_pause(lbm_server *server, int paused);
 
 /* ------------------------------------------------------------------ fields */
