@@ -379,6 +379,7 @@ struct Scene {
   double rhoL;              // liquid density, as set (fresh cells use it)
   int list_cap;
   int has_solid;
+  int live;                 // continues a call that ended after a step's body update
   int use_bf;               // external force armed for the first step of the call
   int cgx, cgy;             // full-grid blocks of 64x4 cells: cgx * cgy * nz
   int bgx, bgy, bgz;        // band blocks of 64x4 cells (covers the largest band box)
@@ -460,9 +461,13 @@ __global__ __launch_bounds__(256) void k1_retag(const Scene sc) {
 // Sorts both lists of the scene's last retag, then runs handleFreshCells; one
 // workgroup per scene.  Upcoming step p: phi is phi[p]; the post-stream
 // populations of fresh cells go to the reference's f (f[1]) before the first
-// step of a call, and in place into f[p] for band updates.
+// step of a call (mode 0, scenes that start from the reference's state), and
+// in place into f[p] for band updates (mode 1, every scene; mode 2, the band
+// update a continued scene owes before its first step).
 template <int KIND>
-__device__ __forceinline__ void sort_fresh_body(const Scene &sc, int p, int prologue) {
+__device__ __forceinline__ void sort_fresh_body(const Scene &sc, int p, int mode) {
+  if ((mode == 0 && sc.live) || (mode == 2 && !sc.live)) return;
+  const int prologue = mode == 0;
   __shared__ int32_t sh[kSortLds];
   int nf = sc.ctrl->n_fresh, nd = sc.ctrl->n_dead;
   if (nf > sc.list_cap) nf = sc.list_cap;
@@ -476,12 +481,12 @@ __device__ __forceinline__ void sort_fresh_body(const Scene &sc, int p, int prol
 }
 template <int KIND>
 __global__ __launch_bounds__(1024) void km_sort_fresh(const Scene *__restrict__ scenes, int p,
-                                                      int prologue) {
-  sort_fresh_body<KIND>(scenes[blockIdx.x], p, prologue);
+                                                      int mode) {
+  sort_fresh_body<KIND>(scenes[blockIdx.x], p, mode);
 }
 template <int KIND>
-__global__ __launch_bounds__(1024) void k1_sort_fresh(const Scene sc, int p, int prologue) {
-  sort_fresh_body<KIND>(sc, p, prologue);
+__global__ __launch_bounds__(1024) void k1_sort_fresh(const Scene sc, int p, int mode) {
+  sort_fresh_body<KIND>(sc, p, mode);
 }
 
 // ---------------------------------------------------------------- collide

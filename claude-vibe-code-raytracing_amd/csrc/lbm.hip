@@ -70,6 +70,10 @@ struct lbm_sim {
   double *partials = nullptr;  // per-block link momentum, 6 doubles per block
   int64_t partial_blocks = 0;
   bool need_full_retag = true;  // next coupled step must retag the whole grid
+  // The last fused call ended after a step's body update, with that step's band
+  // update deferred to the next call (only the serving loop ends calls so; phi,
+  // the tags and the sphere are then the reference's, the other fields not).
+  bool live = false;
   bool sphere_dev_valid = false;
   std::vector<lbm_link> links;  // kept by lbm_stream_hydro_macro_links
   // Compressed populations of the fused step (lbm_set_storage): a pair of
