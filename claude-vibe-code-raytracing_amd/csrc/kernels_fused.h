@@ -829,6 +829,44 @@ __global__ __launch_bounds__(256) void k_mass_partials(int64_t n, Geom g,
 }
 
 
+// Opt-in state check: per 256-cell block, whether phi, rho and u of its fluid
+// cells are finite, and the largest |u| among them (FP64).
+__global__ __launch_bounds__(256) void k_check_partials(int64_t n, const uint8_t *__restrict__ tag,
+                                                        const float *__restrict__ phi,
+                                                        const float *__restrict__ rho,
+                                                        const float *__restrict__ ux,
+                                                        const float *__restrict__ uy,
+                                                        const float *__restrict__ uz,
+                                                        double *__restrict__ partials) {
+  __shared__ double red[4][2];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double bad = 0, um = 0;
+  if (i < n && tag[i] == TAG_FLUID) {
+    const double a = phi[i], r = rho[i], x = ux[i], y = uy[i], z = uz[i];
+    const double u2 = x * x + y * y + z * z;
+    if (!(a == a && r == r && u2 == u2) || fabs(a) == INFINITY || fabs(r) == INFINITY ||
+        u2 == INFINITY)
+      bad = 1;
+    else
+      um = sqrt(u2);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    bad = fmax(bad, __shfl_xor(bad, off, 64));
+    um = fmax(um, __shfl_xor(um, off, 64));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[wave][0] = bad;
+    red[wave][1] = um;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = fmax(fmax(red[0][0], red[1][0]), fmax(red[2][0], red[3][0]));
+    partials[2 * blockIdx.x + 1] = fmax(fmax(red[0][1], red[1][1]), fmax(red[2][1], red[3][1]));
+  }
+}
+
 // ---------------------------------------------------------------- body
 
 // Body update after stream(t) of step q, one 512-thread workgroup per scene:

@@ -850,6 +850,29 @@ extern "C" int lbm_mass(lbm_sim *s, double *phi_sum, double *rho_sum) {
   return LBM_OK;
 }
 
+extern "C" int lbm_check_fields(lbm_sim *s, int *all_finite, double *max_u) {
+  CHECK_SIM(s);
+  const int64_t n = s->g.n;
+  const int64_t blocks = (n + 255) / 256;
+  int rc = ensure_scratch(s, blocks * 2 * sizeof(double));
+  if (rc) return rc;
+  k_check_partials<<<(unsigned)blocks, 256, 0, s->stream>>>(n, s->tag, PHI(s), s->rho, s->ux,
+                                                            s->uy, s->uz, (double *)s->scratch);
+  if ((rc = check_launch(s, "k_check_partials"))) return rc;
+  std::vector<double> part(2 * blocks);
+  HIP_TRY(s, hipMemcpyAsync(part.data(), s->scratch, 2 * blocks * sizeof(double),
+                            hipMemcpyDeviceToHost, s->stream));
+  if ((rc = sync(s))) return rc;
+  double bad = 0, um = 0;
+  for (int64_t k = 0; k < blocks; k++) {
+    bad = std::max(bad, part[2 * k]);
+    um = std::max(um, part[2 * k + 1]);
+  }
+  if (all_finite) *all_finite = bad == 0 ? 1 : 0;
+  if (max_u) *max_u = um;
+  return LBM_OK;
+}
+
 extern "C" int lbm_pack_volume(lbm_sim *s, uint8_t *out) {
   CHECK_SIM(s);
   if (!out) return fail(s, LBM_ERR_ARG, "out is NULL");

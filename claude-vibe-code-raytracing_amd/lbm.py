@@ -131,6 +131,7 @@ def lib():
         "lbm_set_storage": (C.c_int, [P, C.c_int]),
         "lbm_get_storage": (C.c_int, [P]),
         "lbm_mass": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "lbm_check_fields": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
         "lbm_fresh_cells": (C.c_int, [P, C.POINTER(C.c_int32), C.c_int, C.POINTER(C.c_int)]),
         "lbm_dead_cells": (C.c_int, [P, C.POINTER(C.c_int32), C.c_int, C.POINTER(C.c_int)]),
         "lbm_sphere_init": (None, [SP, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -365,6 +366,12 @@ class Sim:
         a, b = C.c_double(), C.c_double()
         self._ck(self.L.lbm_mass(self.h, C.byref(a), C.byref(b)), "lbm_mass")
         return a.value, b.value
+
+    def check_fields(self):
+        """(all finite, max |u|) over fluid cells."""
+        ok, um = C.c_int(), C.c_double()
+        self._ck(self.L.lbm_check_fields(self.h, C.byref(ok), C.byref(um)), "lbm_check_fields")
+        return bool(ok.value), um.value
 
     def pack(self):
         out = np.empty(2 * self.n, np.uint8)

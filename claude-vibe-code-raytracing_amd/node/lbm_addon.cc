@@ -286,6 +286,82 @@ napi_value Step(napi_env env, napi_callback_info info) {
   return undef(env);
 }
 
+// stepAsync(h, sphere | null, nsteps) -> Promise: lbm_step on a worker thread,
+// so the event loop keeps running while the GPU steps (a serving process).
+// The handle and the sphere array stay referenced until the step completes;
+// the caller must not use the simulation meanwhile.
+struct StepWork {
+  napi_async_work work = nullptr;
+  napi_deferred deferred = nullptr;
+  napi_ref handle_ref = nullptr, sphere_ref = nullptr;
+  lbm_sim *s = nullptr;
+  lbm_sphere *b = nullptr;
+  int nsteps = 0;
+  int rc = 0;
+  std::string err;
+};
+
+void step_execute(napi_env, void *data) {
+  StepWork *w = static_cast<StepWork *>(data);
+  w->rc = lbm_step(w->s, w->b, w->nsteps);
+  if (w->rc) w->err = lbm_last_error(w->s);
+}
+
+void step_complete(napi_env env, napi_status, void *data) {
+  StepWork *w = static_cast<StepWork *>(data);
+  if (w->rc == 0) {
+    napi_resolve_deferred(env, w->deferred, undef(env));
+  } else {
+    const std::string msg = "lbm_step failed (" + std::to_string(w->rc) + "): " + w->err;
+    napi_value m, e;
+    napi_create_string_utf8(env, msg.c_str(), msg.size(), &m);
+    napi_create_error(env, nullptr, m, &e);
+    napi_reject_deferred(env, w->deferred, e);
+  }
+  napi_delete_reference(env, w->handle_ref);
+  if (w->sphere_ref) napi_delete_reference(env, w->sphere_ref);
+  napi_delete_async_work(env, w->work);
+  delete w;
+}
+
+napi_value StepAsync(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  lbm_sim *s = get_sim(env, a.v[0]);
+  if (!s) return nullptr;
+  StepWork *w = new StepWork();
+  w->s = s;
+  w->b = get_sphere(env, a.v[1]);
+  w->nsteps = i32(env, a.v[2]);
+  napi_value promise, name;
+  NAPI_OK(env, napi_create_promise(env, &w->deferred, &promise));
+  NAPI_OK(env, napi_create_reference(env, a.v[0], 1, &w->handle_ref));
+  if (w->b) NAPI_OK(env, napi_create_reference(env, a.v[1], 1, &w->sphere_ref));
+  NAPI_OK(env, napi_create_string_utf8(env, "lbm_step", NAPI_AUTO_LENGTH, &name));
+  NAPI_OK(env, napi_create_async_work(env, nullptr, name, step_execute, step_complete, w, &w->work));
+  NAPI_OK(env, napi_queue_async_work(env, w->work));
+  return promise;
+}
+
+// checkFields(h) -> [allFinite, maxU]
+napi_value CheckFields(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  lbm_sim *s = get_sim(env, a.v[0]);
+  if (!s) return nullptr;
+  int ok = 0;
+  double um = 0;
+  const int rc = lbm_check_fields(s, &ok, &um);
+  if (rc) return throw_status(env, s, rc, "lbm_check_fields");
+  napi_value arr, v0, v1;
+  NAPI_OK(env, napi_create_array_with_length(env, 2, &arr));
+  NAPI_OK(env, napi_get_boolean(env, ok != 0, &v0));
+  NAPI_OK(env, napi_create_double(env, um, &v1));
+  NAPI_OK(env, napi_set_element(env, arr, 0, v0));
+  NAPI_OK(env, napi_set_element(env, arr, 1, v1));
+  return arr;
+}
+
 // download(h, field, typedArray) / upload(h, field, typedArray)
 napi_value Transfer(napi_env env, napi_callback_info info, bool down) {
   Args a;
@@ -485,6 +561,8 @@ NAPI_MODULE_INIT() {
       FN("simStep", SimStep),
       FN("absorbDead", AbsorbDead),
       FN("step", Step),
+      FN("stepAsync", StepAsync),
+      FN("checkFields", CheckFields),
       FN("download", Download),
       FN("upload", Upload),
       FN("pack", Pack),

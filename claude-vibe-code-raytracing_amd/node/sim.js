@@ -212,6 +212,25 @@ export class Sim {
     this._call(() => native.step(this._h, solid ? solid.state : null, nsteps));
   }
 
+  // run() on a worker thread: resolves when the steps are done, with the
+  // mirrors refreshed.  The Sim must not be used until then.  Generic solids
+  // step synchronously (their callbacks run on this thread).
+  runAsync(solid, nsteps) {
+    if (solid != null && !(solid instanceof RigidSphere)) {
+      this.run(solid, nsteps);
+      return Promise.resolve();
+    }
+    this._push();
+    return native.stepAsync(this._h, solid ? solid.state : null, nsteps).then(() => this._pull());
+  }
+
+  // Opt-in state check (not on the hot path): { finite, maxU } over fluid cells.
+  checkFields() {
+    this._push();
+    const [finite, maxU] = native.checkFields(this._h);
+    return { finite, maxU };
+  }
+
   // RigidSphere.absorbDeadCells(sim) for a device sphere.
   _absorbDead(solid) { this._call(() => native.absorbDead(this._h, solid.state)); }
 

@@ -198,6 +198,10 @@ int lbm_get_storage(const lbm_sim *sim);
 /* Global mass check: FP64 sums of phi and of rho (sum f) over fluid cells,
  * added in a fixed order.  No reference counterpart. */
 int lbm_mass(lbm_sim *sim, double *phi_sum, double *rho_sum);
+/* Opt-in state check, off the hot path: *all_finite = 1 when phi, rho and u are
+ * finite at every fluid cell, *max_u the largest |u| there.  The reference has
+ * no error paths; its callers scan for NaN (test/smoke.mjs:33-40). */
+int lbm_check_fields(lbm_sim *sim, int *all_finite, double *max_u);
 
 /* ------------------------------------------------------------------ batches */
 /* Many simulations stepped together.  No reference counterpart: the reference

@@ -95,3 +95,10 @@ def test_frame_delivery_end_to_end():
     r = run_node("frame_delivery.mjs", timeout=120)
     assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-2000:])
     assert "OK frames=" in r.stdout
+
+
+def test_dropin_run_async_and_check():
+    """runAsync (worker thread, Promise) equals run; checkFields on the state."""
+    r = run_node("async_run.mjs", timeout=120)
+    assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-2000:])
+    assert "ASYNC OK" in r.stdout

@@ -342,3 +342,22 @@ def test_fused_vs_oracle_larger_grid():
     for k in ("phi", "rho", "ux", "uy", "uz"):
         e = rel_err(sg.get(k), getattr(so, k), mask)
         assert e <= 1e-5, (k, e)
+
+
+def test_check_fields_flags_nan():
+    """lbm_check_fields: finite and |u| <= 0.2 after steps; a NaN written into a
+    fluid cell's phi is reported."""
+    announce("check fields")
+    sim = lbm.Sim(40, 28, 40)
+    b = scenes.config0(sim, gpu_sphere)
+    sim.coupled(b, 20)
+    ok, um = sim.check_fields()
+    assert ok and 0 < um <= 0.2 + 1e-7
+    phi = sim.get("phi")
+    tag = sim.get("tag")
+    i = int(np.flatnonzero(tag == 0)[100])
+    phi[i] = np.nan
+    sim.set("phi", phi)
+    ok, _ = sim.check_fields()
+    assert not ok
+    sim.close()
