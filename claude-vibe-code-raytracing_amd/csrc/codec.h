@@ -29,6 +29,18 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float v) {
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
+// Element of an array at a per-lane byte offset from a wave-uniform base (an
+// array, or a plane shifted by a uniform pull offset).  Byte offsets stay below
+// 4 GiB: grids hold fewer than 2^30 cells.
+template <typename T>
+__device__ __forceinline__ T ldb(const T *base, uint32_t bo) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + bo);
+}
+template <typename T>
+__device__ __forceinline__ void stb(T *base, uint32_t bo, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + bo) = v;
+}
+
 template <int KIND>
 struct Codec;
 
@@ -40,6 +52,9 @@ struct Codec<ST_F32> {
   static constexpr int bytes = 4;
   __device__ __forceinline__ static float ld(const T *p, int64_t a, float) { return p[a]; }
   __device__ __forceinline__ static void st(T *p, int64_t a, float v, float) { p[a] = v; }
+  // ld / st at a byte offset from a uniform base (ldb / stb).
+  __device__ __forceinline__ static float ldo(const T *p, uint32_t bo, float) { return ldb(p, bo); }
+  __device__ __forceinline__ static void sto(T *p, uint32_t bo, float v, float) { stb(p, bo, v); }
   // The value st then ld give back.
   __device__ __forceinline__ static float rt(float v, float) { return v; }
 };
@@ -55,9 +70,32 @@ struct Codec<ST_F16> {
   __device__ __forceinline__ static void st(T *p, int64_t a, float v, float base) {
     p[a] = (T)(v - base);
   }
+  __device__ __forceinline__ static float ldo(const T *p, uint32_t bo, float base) {
+    return (float)ldb(p, bo) + base;
+  }
+  __device__ __forceinline__ static void sto(T *p, uint32_t bo, float v, float base) {
+    stb(p, bo, (T)(v - base));
+  }
   __device__ __forceinline__ static float rt(float v, float base) {
     return (float)(T)(v - base) + base;
   }
+  // The stored bits of an element: dec(enc(v, b), b) == ld after st.
+  __device__ __forceinline__ static uint16_t enc(float v, float base) {
+    return __builtin_bit_cast(uint16_t, (T)(v - base));
+  }
+  __device__ __forceinline__ static float dec(uint16_t bits, float base) {
+    return (float)__builtin_bit_cast(T, bits) + base;
+  }
+  // Deviations themselves (the stored value, no baseline added): dv of the
+  // stored bits, ev the bits of a deviation, rtv a deviation as stored and
+  // reloaded; ldd / std load and store a deviation at a byte offset.
+  __device__ __forceinline__ static float dv(uint16_t bits) {
+    return (float)__builtin_bit_cast(T, bits);
+  }
+  __device__ __forceinline__ static uint16_t ev(float v) { return __builtin_bit_cast(uint16_t, (T)v); }
+  __device__ __forceinline__ static float rtv(float v) { return (float)(T)v; }
+  __device__ __forceinline__ static float ldd(const T *p, uint32_t bo) { return (float)ldb(p, bo); }
+  __device__ __forceinline__ static void std_(T *p, uint32_t bo, float v) { stb(p, bo, (T)v); }
 };
 
 template <>
@@ -71,9 +109,24 @@ struct Codec<ST_BF16> {
   __device__ __forceinline__ static void st(T *p, int64_t a, float v, float base) {
     p[a] = f32_to_bf16(v - base);
   }
+  __device__ __forceinline__ static float ldo(const T *p, uint32_t bo, float base) {
+    return bf16_to_f32(ldb(p, bo)) + base;
+  }
+  __device__ __forceinline__ static void sto(T *p, uint32_t bo, float v, float base) {
+    stb(p, bo, f32_to_bf16(v - base));
+  }
   __device__ __forceinline__ static float rt(float v, float base) {
     return bf16_to_f32(f32_to_bf16(v - base)) + base;
   }
+  __device__ __forceinline__ static uint16_t enc(float v, float base) { return f32_to_bf16(v - base); }
+  __device__ __forceinline__ static float dec(uint16_t bits, float base) {
+    return bf16_to_f32(bits) + base;
+  }
+  __device__ __forceinline__ static float dv(uint16_t bits) { return bf16_to_f32(bits); }
+  __device__ __forceinline__ static uint16_t ev(float v) { return f32_to_bf16(v); }
+  __device__ __forceinline__ static float rtv(float v) { return bf16_to_f32(f32_to_bf16(v)); }
+  __device__ __forceinline__ static float ldd(const T *p, uint32_t bo) { return bf16_to_f32(ldb(p, bo)); }
+  __device__ __forceinline__ static void std_(T *p, uint32_t bo, float v) { stb(p, bo, f32_to_bf16(v)); }
 };
 
 }  // namespace lbm

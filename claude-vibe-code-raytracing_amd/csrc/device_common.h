@@ -76,12 +76,29 @@ __device__ __forceinline__ bool map_tile(int T, int T8, int lb, int &tile) {
   return tile < T;
 }
 
-// Cell coordinates of index i.
+// Quotient and remainder of a / b for 0 <= a < 2^31 and 1 <= b, when the
+// quotient is below 2^20: the float estimate is then within one of it, and
+// one correction each way makes it exact.
+__device__ __forceinline__ int small_divmod(int a, int b, int &rem) {
+  int q = (int)((float)a * __builtin_amdgcn_rcpf((float)b));
+  int r = a - q * b;
+  if (r < 0) {
+    q -= 1;
+    r += b;
+  }
+  if (r >= b) {
+    q += 1;
+    r -= b;
+  }
+  rem = r;
+  return q;
+}
+
+// Cell coordinates of index i (grids of fewer than 2^31 cells, axes below 2^20).
 __device__ __forceinline__ void cell_xyz(int nx, int nxny, int64_t i, int &x, int &y, int &z) {
-  z = (int)(i / nxny);
-  const int r = (int)(i - (int64_t)z * nxny);
-  y = r / nx;
-  x = r - y * nx;
+  int r;
+  z = small_divmod((int)i, nxny, r);
+  y = small_divmod(r, nx, x);
 }
 
 // True when tile t holds a cell of the band box's rows: in some box plane z the

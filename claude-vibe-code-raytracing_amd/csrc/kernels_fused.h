@@ -32,6 +32,8 @@
 // the reference's collide(t+1) reads.
 #pragma once
 
+#include <type_traits>
+
 #include "codec.h"
 #include "device_common.h"
 #include "kernels_pass.h"
@@ -63,6 +65,11 @@ __host__ __device__ inline void band_box(const lbm_sphere &b, int nx, int ny, in
   }
 }
 
+// Pull offset of direction k: the source of cell i is i - dir_off(g, k).
+__device__ __forceinline__ int64_t dir_off(const Geom &g, int k) {
+  return (int64_t)EZ27[k] * g.nxny + (int64_t)EY27[k] * g.nx + EX27[k];
+}
+
 __device__ __forceinline__ bool in_box(const int box[6], int x, int y, int z) {
   return x >= box[0] && x <= box[1] && y >= box[2] && y <= box[3] && z >= box[4] && z <= box[5];
 }
@@ -70,16 +77,42 @@ __device__ __forceinline__ bool in_box(const int box[6], int x, int y, int z) {
 // Gradient and Laplacian of a phi array at interior cell i (src/sim.js:186-195),
 // rounded to float as the reference's gx/gy/gz/lap arrays store them.  R is the
 // arithmetic type (double for the reference-exact path).
+// phi of a cell and its six face neighbours.
+struct Phi7 {
+  float c, px, mx, py, my, pz, mz;
+  // phi of the source of D3Q7 direction k (the cell itself for k = 0).
+  __device__ __forceinline__ float src(int k) const {
+    return k == 0 ? c : k == 1 ? mx : k == 2 ? px : k == 3 ? my : k == 4 ? py : k == 5 ? mz : pz;
+  }
+};
+// bo4: byte offset of the cell in a float array.
+__device__ __forceinline__ Phi7 load_phi7(const Geom &g, const float *__restrict__ phi,
+                                          uint32_t bo4) {
+  Phi7 v;
+  v.c = ldb(phi, bo4);
+  v.px = ldb(phi + 1, bo4);
+  v.mx = ldb(phi - 1, bo4);
+  v.py = ldb(phi + g.nx, bo4);
+  v.my = ldb(phi - g.nx, bo4);
+  v.pz = ldb(phi + g.nxny, bo4);
+  v.mz = ldb(phi - g.nxny, bo4);
+  return v;
+}
 template <typename R>
-__device__ __forceinline__ void phi_grad(const Geom &g, const float *__restrict__ phi, int64_t i,
-                                         R c, float &gxo, float &gyo, float &gzo, float &lapo) {
-  const R px = phi[i + 1], mx = phi[i - 1];
-  const R py = phi[i + g.nx], my = phi[i - g.nx];
-  const R pz = phi[i + g.nxny], mz = phi[i - g.nxny];
+__device__ __forceinline__ void phi_grad7(const Phi7 &v, R c, float &gxo, float &gyo, float &gzo,
+                                          float &lapo) {
+  const R px = v.px, mx = v.mx;
+  const R py = v.py, my = v.my;
+  const R pz = v.pz, mz = v.mz;
   gxo = (float)(R(0.5) * (px - mx));
   gyo = (float)(R(0.5) * (py - my));
   gzo = (float)(R(0.5) * (pz - mz));
   lapo = (float)(px + mx + py + my + pz + mz - 6 * c);
+}
+template <typename R>
+__device__ __forceinline__ void phi_grad(const Geom &g, const float *__restrict__ phi, int64_t i,
+                                         R c, float &gxo, float &gyo, float &gzo, float &lapo) {
+  phi_grad7<R>(load_phi7(g, phi, (uint32_t)i * 4u), c, gxo, gyo, gzo, lapo);
 }
 
 // e_k . v for a lattice direction with components in {-1, 0, 1}.  The reference
@@ -189,6 +222,115 @@ __device__ __forceinline__ void collide_hydro_cell(const Derived &d, R sumF, R p
   HydroUnroll<Q27, R>::run(d, sumF, rhoPhase, negU2half, uF, uxe, uye, uze, fx, fy, fz, fin, fout);
 }
 
+// ---------------------------------------------------------------- 16-bit codecs: deviation form
+//
+// With 16-bit storage the step computes in FP32 on the stored deviations
+// themselves: d_k = f_k - w_k, and h_k - w7_k phi with phi that of the cell the
+// population is stored at.  A pulled f deviation is exactly the stored value,
+// so decoding is one conversion and a post-stream population goes back to
+// storage without loss; the momentum needs no reference level (w_k e_k sums to
+// zero over each opposite pair); the equilibrium enters as its deviation
+// w_k (rho - 1 + rho_phase Q_k), and products and sums are fused (fmaf) where
+// that saves an instruction.  The results are not the float32 form's bit for
+// bit (that form is FP64 on full values) but are deterministic, and the fused
+// step and a call split between stream and collide store and reload the same
+// 16-bit values.  D3Q27 directions come in opposite pairs (2j - 1, 2j), so e.v
+// is formed once per pair.
+
+// Hydro collide in deviation form (collideHydro, src/sim.js:430-459): din and
+// dout are f_k - w_k; rhof is rho as stored (sum f).
+template <int K>
+__device__ __forceinline__ void hydro_dev_pair(float sdev, float rhoPhase, float nu2h, float uF,
+                                               float ux, float uy, float uz, float fx, float fy,
+                                               float fz, float it, float A, const float din[Q27],
+                                               float dout[Q27]) {
+  // K: the first direction of a pair; K + 1 is its opposite.
+  constexpr int ex = EX27[K], ey = EY27[K], ez = EZ27[K];
+  const float wk = w27(K);  // == w27(K + 1)
+  const float Awk = A * wk;
+  const float eu = edot<ex, ey, ez>(ux, uy, uz);
+  const float ekF = edot<ex, ey, ez>(fx, fy, fz);
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const float e = j ? -eu : eu, eF = j ? -ekF : ekF;
+    const float din_k = din[K + j];
+    const float Qi = fmaf(e, fmaf(float(INV_2CS4_27), e, float(INV_CS2_27)), nu2h);
+    const float diff = fmaf(wk, fmaf(rhoPhase, Qi, sdev), -din_k);
+    const float o = fmaf(it, diff, din_k);
+    const float S = fmaf(eF, fmaf(float(INV_CS2_27), e, 1.0f), -uF);
+    dout[K + j] = fmaf(Awk, S, o);
+  }
+}
+template <int K>
+__device__ __forceinline__ void hydro_dev_pairs(float sdev, float rhoPhase, float nu2h, float uF,
+                                                float ux, float uy, float uz, float fx, float fy,
+                                                float fz, float it, float A, const float din[Q27],
+                                                float dout[Q27]) {
+  if constexpr (K < Q27) {
+    hydro_dev_pair<K>(sdev, rhoPhase, nu2h, uF, ux, uy, uz, fx, fy, fz, it, A, din, dout);
+    hydro_dev_pairs<K + 2>(sdev, rhoPhase, nu2h, uF, ux, uy, uz, fx, fy, fz, it, A, din, dout);
+  }
+}
+__device__ __forceinline__ void collide_hydro_dev(const Derived &d, float rhof, float p, float lap,
+                                                  float ggx, float ggy, float ggz, float bx,
+                                                  float by, float bz, float ux, float uy, float uz,
+                                                  const float din[Q27], float dout[Q27]) {
+  const float rhoPhase = (float)d.rhoG + p * (float)d.dRho;
+  const float mu = 4 * (float)d.beta * p * (p - 1) * (2 * p - 1) - (float)d.kappa * lap;
+  const float fx = bx + mu * ggx;
+  const float fy = by + mu * ggy + rhoPhase * (float)d.gravity;
+  const float fz = bz + mu * ggz;
+  const float nu2h = -1.5f * (ux * ux + uy * uy + uz * uz);
+  const float uF = ux * fx + uy * fy + uz * fz;
+  const float sdev = rhof - 1.0f;
+  const float it = (float)d.invTau;
+  // k = 0: e = 0, so Q = -1.5 u^2 and the forcing term is -A w0 u.F.
+  {
+    const float w0 = w27(0);
+    const float diff = fmaf(w0, fmaf(rhoPhase, nu2h, sdev), -din[0]);
+    dout[0] = fmaf((float)d.A * w0, -uF, fmaf(it, diff, din[0]));
+  }
+  hydro_dev_pairs<1>(sdev, rhoPhase, nu2h, uF, ux, uy, uz, fx, fy, fz, it, (float)d.A, din, dout);
+}
+
+// Phase collide in deviation form (collidePhase, src/sim.js:341-360): hin and
+// hout are h_k - w7_k p, p the phi of the cell (the collide keeps it), so
+// h_eq - w7_k p = w7_k p e_k.u / cs^2.
+__device__ __forceinline__ void collide_phase_dev(const Derived &d, float p, float ux, float uy,
+                                                  float uz, float ggx, float ggy, float ggz,
+                                                  const float hin[Q7], float hout[Q7]) {
+  const float gmag = sqrtf(ggx * ggx + ggy * ggy + ggz * ggz) + 1e-12f;
+  const float sf = (float)d.sharpC * p * (1 - p) * (float)d.sharpFac / gmag;
+  const float S[3] = {sf * ggx, sf * ggy, sf * ggz};
+  const float u[3] = {ux, uy, uz};
+  const float it = (float)d.invTauPhi;
+  const float a = w7(1) * p * float(INV_CS2_7);
+  hout[0] = fmaf(it, -hin[0], hin[0]);
+#pragma unroll
+  for (int k = 1; k < Q7; k++) {
+    const int ax = (k - 1) >> 1;
+    const float sg = (k & 1) ? 1.0f : -1.0f;
+    const float diff = fmaf(a, sg * u[ax], -hin[k]);
+    hout[k] = fmaf(w7(k) * sg, S[ax], fmaf(it, diff, hin[k]));
+  }
+}
+
+// phi(t+1) of a fluid cell from its pulled h deviations e and their baselines'
+// phi (the source's, or the cell's own for a bounced population): the full
+// populations, summed in direction order, and their sum clamped to [0, 1].
+// The phase stream and the stream kernel both use it, so both write the same
+// phi(t+1).
+__device__ __forceinline__ float phase_sum_dev(const float e[Q7], const float phib[Q7],
+                                               float hfull[Q7]) {
+  float sum = 0.0f;
+#pragma unroll
+  for (int k = 0; k < Q7; k++) {
+    hfull[k] = fmaf(w7(k), phib[k], e[k]);
+    sum += hfull[k];
+  }
+  return sum < 0.0f ? 0.0f : sum > 1.0f ? 1.0f : sum;
+}
+
 // Direction masks: bit k set when the component of e_k along the axis is
 // positive (P) or negative (M).
 template <int A, int SIGN>
@@ -226,44 +368,30 @@ __device__ __forceinline__ void tag_masks(const Geom &g, int64_t i,
                                           uint32_t &solid) {
   nonfluid = 0;
   solid = 0;
+  const uint32_t bo = (uint32_t)i;
 #pragma unroll
   for (int k = 1; k < Q27; k++) {
-    const int64_t s = i - ((int64_t)EZ27[k] * g.nxny + (int64_t)EY27[k] * g.nx + EX27[k]);
-    const uint8_t st = tag[s];
+    const uint8_t st = ldb(tag - ((int64_t)EZ27[k] * g.nxny + (int64_t)EY27[k] * g.nx + EX27[k]), bo);
     nonfluid |= (uint32_t)(st != TAG_FLUID) << k;
     solid |= (uint32_t)(st == TAG_SOLID) << k;
   }
 }
 
-// Pull stream of the 27 populations with halfway bounce-back
-// (src/sim.js:509-543), given the source masks.  Every load address is chosen
-// without branching (own opposite population at a non-fluid source), so the 27
-// loads are in flight together; the moving-wall term and the momentum
-// exchange run only for solid links.  fk stays the unrounded value the
-// reference sums and exchanges; fp receives it rounded to float.  The momentum
-// exchange accumulates in FP64 whatever the arithmetic type.
-template <int KIND>
-__device__ __forceinline__ void pull27(const Geom &g, int64_t i,
-                                       const typename Codec<KIND>::T *__restrict__ f2,
-                                       uint32_t nonfluid, uint32_t solid,
-                                       const float *__restrict__ usx,
-                                       const float *__restrict__ usy,
-                                       const float *__restrict__ usz,
-                                       typename Codec<KIND>::R rhoPhaseOld, bool exchange, int x,
-                                       int y, int z, double cx, double cy, double cz,
-                                       double mom[6], float fp[Q27],
-                                       typename Codec<KIND>::R &pSumF,
-                                       typename Codec<KIND>::R &mx, typename Codec<KIND>::R &my,
-                                       typename Codec<KIND>::R &mz) {
-  using C = Codec<KIND>;
-  using R = typename C::R;
-  float raw[Q27];
-#pragma unroll
-  for (int k = 0; k < Q27; k++) {
-    const int64_t s = i - ((int64_t)EZ27[k] * g.nxny + (int64_t)EY27[k] * g.nx + EX27[k]);
-    const int64_t a = ((nonfluid >> k) & 1u) ? opp27(k) * g.S + i : k * g.S + s;
-    raw[k] = C::ld(f2, a, w27(k));  // w27(opp(k)) == w27(k)
-  }
+// Halfway bounce-back and momentum exchange of the 27 pulled populations
+// (src/sim.js:509-543): raw[k] is the population pulled from the source cell,
+// or the cell's own opposite population where the source is not fluid; the
+// moving-wall term and the momentum exchange run only for solid links.  fk
+// stays the unrounded value the reference sums and exchanges; fp receives it
+// rounded to float.  The momentum exchange accumulates in FP64 whatever the
+// arithmetic type R.
+template <typename R>
+__device__ __forceinline__ void links27(const Geom &g, int64_t i, const float raw[Q27],
+                                        uint32_t solid, const float *__restrict__ usx,
+                                        const float *__restrict__ usy,
+                                        const float *__restrict__ usz, R rhoPhaseOld,
+                                        bool exchange, int x, int y, int z, double cx, double cy,
+                                        double cz, double mom[6], float fp[Q27], R &pSumF, R &mx,
+                                        R &my, R &mz) {
   pSumF = 0;
   mx = my = mz = 0;
 #pragma unroll
@@ -304,24 +432,118 @@ __device__ __forceinline__ void pull27(const Geom &g, int64_t i,
   }
 }
 
+// Pull stream of the 27 populations with halfway bounce-back, given the source
+// masks: every population is pulled from its source (inside the grid for an
+// interior cell), all 27 loads in flight together; lanes with a non-fluid
+// source then take their own opposite populations instead.
+template <int KIND>
+__device__ __forceinline__ void pull27(const Geom &g, int64_t i,
+                                       const typename Codec<KIND>::T *__restrict__ f2,
+                                       uint32_t nonfluid, uint32_t solid,
+                                       const float *__restrict__ usx,
+                                       const float *__restrict__ usy,
+                                       const float *__restrict__ usz,
+                                       typename Codec<KIND>::R rhoPhaseOld, bool exchange, int x,
+                                       int y, int z, double cx, double cy, double cz,
+                                       double mom[6], float fp[Q27],
+                                       typename Codec<KIND>::R &pSumF,
+                                       typename Codec<KIND>::R &mx, typename Codec<KIND>::R &my,
+                                       typename Codec<KIND>::R &mz) {
+  using C = Codec<KIND>;
+  using T = typename C::T;
+  const uint32_t bo = (uint32_t)i * (uint32_t)sizeof(T);
+  float raw[Q27];
+#pragma unroll
+  for (int k = 0; k < Q27; k++) {
+    const int64_t off = (int64_t)EZ27[k] * g.nxny + (int64_t)EY27[k] * g.nx + EX27[k];
+    raw[k] = C::ldo(f2 + ((int64_t)k * g.S - off), bo, w27(k));
+  }
+  if (nonfluid) {
+#pragma unroll
+    for (int k = 1; k < Q27; k++)
+      if ((nonfluid >> k) & 1u)
+        raw[k] = C::ldo(f2 + (int64_t)opp27(k) * g.S, bo, w27(k));  // w27(opp(k)) == w27(k)
+  }
+  links27<typename C::R>(g, i, raw, solid, usx, usy, usz, rhoPhaseOld, exchange, x, y, z, cx, cy,
+                         cz, mom, fp, pSumF, mx, my, mz);
+}
+
+// links27 in deviation form (16-bit codecs): raw holds the pulled deviations
+// (the cell's own opposite ones at non-fluid sources; w_k is the same for a
+// direction and its opposite).  fp receives the collide's input, each
+// deviation as storing and reloading it gives it back (only a corrected one
+// changes); sumD and the momentum sum the unrounded deviations.
+template <int KIND>
+__device__ __forceinline__ void links27_dev(const Geom &g, int64_t i, const float raw[Q27],
+                                            uint32_t solid, const float *__restrict__ usx,
+                                            const float *__restrict__ usy,
+                                            const float *__restrict__ usz, float rhoPhaseOld,
+                                            bool exchange, int x, int y, int z, double cx,
+                                            double cy, double cz, double mom[6], float fp[Q27],
+                                            float &sumD, float &mx, float &my, float &mz) {
+  using C = Codec<KIND>;
+  sumD = 0;
+  mx = my = mz = 0;
+#pragma unroll
+  for (int k = 0; k < Q27; k++) {
+    float dk = raw[k];
+    float in = dk;
+    if ((solid >> k) & 1u) {
+      const int ex = EX27[k], ey = EY27[k], ez = EZ27[k];
+      const int64_t s = i - ((int64_t)ez * g.nxny + (int64_t)ey * g.nx + ex);
+      const float dopp = dk;
+      const float eu = (float)ex * usx[s] + (float)ey * usy[s] + (float)ez * usz[s];
+      dk = dopp + 2 * w27(k) * rhoPhaseOld * eu * float(INV_CS2_27);
+      in = C::rtv(dk);
+      if (exchange) {
+        // applyImpulseAtCell(x - ex, y - ey, z - ez, -m ex, -m ey, -m ez), m the
+        // sum of the two full populations
+        const double m = 2.0 * (double)w27(k) + ((double)dopp + (double)dk);
+        const double px = -m * ex, py = -m * ey, pz = -m * ez;
+        const double rx = (x - ex) + 0.5 - cx;
+        const double ry = (y - ey) + 0.5 - cy;
+        const double rz = (z - ez) + 0.5 - cz;
+        mom[0] += px;
+        mom[1] += py;
+        mom[2] += pz;
+        mom[3] += ry * pz - rz * py;
+        mom[4] += rz * px - rx * pz;
+        mom[5] += rx * py - ry * px;
+      }
+    }
+    fp[k] = in;
+    sumD += dk;
+    if (EX27[k] > 0) mx += dk;
+    if (EX27[k] < 0) mx -= dk;
+    if (EY27[k] > 0) my += dk;
+    if (EY27[k] < 0) my -= dk;
+    if (EZ27[k] > 0) mz += dk;
+    if (EZ27[k] < 0) mz -= dk;
+  }
+}
+
 // Pull of the 7 phase populations (src/sim.js:376-397): from the source cell,
 // or the own opposite population when the source is not fluid (D3Q7 directions
 // 1..6 are D3Q27 directions 1..6, so the D3Q27 source mask serves).  A stored h
-// carries the phi of the cell it is stored at as its baseline.
+// carries the phi of the cell it is stored at as its baseline: v holds phi of
+// the cell and its face neighbours (read only for 16-bit codecs).
 template <int KIND>
 __device__ __forceinline__ void pull7(const Geom &g, int64_t i,
                                       const typename Codec<KIND>::T *__restrict__ h2,
-                                      const float *__restrict__ phi, uint32_t nonfluid,
-                                      float hp[Q7]) {
+                                      const Phi7 &v, uint32_t nonfluid, float hp[Q7]) {
   using C = Codec<KIND>;
+  using T = typename C::T;
+  const uint32_t bo = (uint32_t)i * (uint32_t)sizeof(T);
 #pragma unroll
   for (int k = 0; k < Q7; k++) {
-    const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
-    const bool bounce = (nonfluid >> k) & 1u;
-    const int64_t a = bounce ? opp7(k) * g.S + i : k * g.S + s;
-    float base = 0.0f;
-    if (KIND != ST_F32) base = w7(k) * phi[bounce ? i : s];
-    hp[k] = C::ld(h2, a, base);
+    const int64_t off = (int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k];
+    hp[k] = C::ldo(h2 + ((int64_t)k * g.S - off), bo, KIND != ST_F32 ? w7(k) * v.src(k) : 0.0f);
+  }
+  if (nonfluid & 0x7eu) {
+#pragma unroll
+    for (int k = 1; k < Q7; k++)
+      if ((nonfluid >> k) & 1u)
+        hp[k] = C::ldo(h2 + (int64_t)opp7(k) * g.S, bo, KIND != ST_F32 ? w7(k) * v.c : 0.0f);
   }
 }
 
@@ -517,15 +739,39 @@ __device__ __forceinline__ void collide_body(const Scene &sc, int lb, int p) {
   float gx1, gy1, gz1, lap1;
   phi_grad<R>(g, phi, i, pc, gx1, gy1, gz1, lap1);
   const R uxe = sc.ux[i], uye = sc.uy[i], uze = sc.uz[i];
+  const uint32_t bo = (uint32_t)i * (uint32_t)sizeof(T);
+  if constexpr (KIND != ST_F32) {
+    // Deviation form: h is stored against w7_k phi of this cell, f against w_k.
+    const uint32_t bo4 = (uint32_t)i * 4u;
+    const float uxf = ldb(sc.ux, bo4), uyf = ldb(sc.uy, bo4), uzf = ldb(sc.uz, bo4);
+    float hin[Q7], hc[Q7];
+#pragma unroll
+    for (int k = 0; k < Q7; k++) hin[k] = C::ldd(hi + (int64_t)k * g.S, bo);
+    collide_phase_dev(sc.d, phc, uxf, uyf, uzf, gx1, gy1, gz1, hin, hc);
+#pragma unroll
+    for (int k = 0; k < Q7; k++) C::std_(ho + (int64_t)k * g.S, bo, hc[k]);
+    float fin[Q27], fc[Q27];
+#pragma unroll
+    for (int k = 0; k < Q27; k++) fin[k] = C::ldd(fi + (int64_t)k * g.S, bo);
+    const bool bf = !BAND && sc.use_bf;
+    const float bx = bf ? ldb(sc.bfx, bo4) : 0.0f;
+    const float by = bf ? ldb(sc.bfy, bo4) : 0.0f;
+    const float bz = bf ? ldb(sc.bfz, bo4) : 0.0f;
+    collide_hydro_dev(sc.d, ldb(sc.rho, bo4), phc, lap1, gx1, gy1, gz1, bx, by, bz, uxf, uyf, uzf,
+                      fin, fc);
+#pragma unroll
+    for (int k = 0; k < Q27; k++) C::std_(fo + (int64_t)k * g.S, bo, fc[k]);
+    return;
+  }
   float hin[Q7], hc[Q7];
 #pragma unroll
-  for (int k = 0; k < Q7; k++) hin[k] = C::ld(hi, k * g.S + i, w7(k) * phc);
+  for (int k = 0; k < Q7; k++) hin[k] = C::ldo(hi + (int64_t)k * g.S, bo, w7(k) * phc);
   collide_phase_cell<R>(sc.d, pc, uxe, uye, uze, (R)gx1, (R)gy1, (R)gz1, hin, hc);
 #pragma unroll
-  for (int k = 0; k < Q7; k++) C::st(ho, k * g.S + i, hc[k], w7(k) * phc);
+  for (int k = 0; k < Q7; k++) C::sto(ho + (int64_t)k * g.S, bo, hc[k], w7(k) * phc);
   float fin[Q27], fc[Q27];
 #pragma unroll
-  for (int k = 0; k < Q27; k++) fin[k] = C::ld(fi, k * g.S + i, w27(k));
+  for (int k = 0; k < Q27; k++) fin[k] = C::ldo(fi + (int64_t)k * g.S, bo, w27(k));
   const bool bf = !BAND && sc.use_bf;
   const R bx = bf ? (R)sc.bfx[i] : R(0);
   const R by = bf ? (R)sc.bfy[i] : R(0);
@@ -533,7 +779,7 @@ __device__ __forceinline__ void collide_body(const Scene &sc, int lb, int p) {
   collide_hydro_cell<R>(sc.d, (R)sc.rho[i], pc, (R)lap1, (R)gx1, (R)gy1, (R)gz1, bx, by, bz, uxe,
                         uye, uze, fin, fc);
 #pragma unroll
-  for (int k = 0; k < Q27; k++) C::st(fo, k * g.S + i, fc[k], w27(k));
+  for (int k = 0; k < Q27; k++) C::sto(fo + (int64_t)k * g.S, bo, fc[k], w27(k));
 }
 template <bool BAND, int KIND>
 __global__ __launch_bounds__(256) void km_collide(const Scene *__restrict__ scenes,
@@ -573,27 +819,51 @@ __device__ __forceinline__ void phase_stream_body(const Scene &sc, int lb, int q
 #pragma unroll
   for (int j = 0; j < 6; j++) box[j] = sc.ctrl->box_fused[j];
   const bool near = in_box(box, x, y, z);
-  if (!interior(g, x, y, z) || (near && tag[i] != TAG_FLUID)) {
-    phi_next[i] = phi[i];
+  const uint32_t bo4 = (uint32_t)i * 4u;
+  if (!interior(g, x, y, z) || (near && ldb(tag, (uint32_t)i) != TAG_FLUID)) {
+    stb(phi_next, bo4, ldb(phi, bo4));
     return;
   }
   uint32_t nonfluid = 0;
   if (near) {
 #pragma unroll
     for (int k = 1; k < Q7; k++) {
-      const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
-      nonfluid |= (uint32_t)(tag[s] != TAG_FLUID) << k;
+      const int64_t off = (int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k];
+      nonfluid |= (uint32_t)(ldb(tag - off, (uint32_t)i) != TAG_FLUID) << k;
     }
   } else {
     nonfluid = wall_mask(g, x, y, z) & 0x7eu;  // D3Q7 directions 1..6
   }
-  float hv[Q7];
-  pull7<KIND>(g, i, h2, phi, nonfluid, hv);
-  R phiSum = 0;
+  if constexpr (KIND != ST_F32) {
+    // The stream kernel's phi(t+1) of the cell (phase_sum_dev), from the same
+    // pulled deviations and baselines.
+    using C = Codec<KIND>;
+    const Phi7 v = load_phi7(g, phi, bo4);
+    const uint32_t bo = (uint32_t)i * 2u;
+    float e[Q7], phib[Q7], hfull[Q7];
 #pragma unroll
-  for (int k = 0; k < Q7; k++) phiSum += (R)hv[k];
-  const R pp = phiSum < 0 ? R(0) : phiSum > 1 ? R(1) : phiSum;
-  phi_next[i] = (float)pp;
+    for (int k = 0; k < Q7; k++) {
+      e[k] = C::ldd(h2 + ((int64_t)k * g.S - dir_off(g, k)), bo);
+      phib[k] = v.src(k);
+    }
+    if (nonfluid) {
+#pragma unroll
+      for (int k = 1; k < Q7; k++)
+        if ((nonfluid >> k) & 1u) {
+          e[k] = C::ldd(h2 + (int64_t)opp7(k) * g.S, bo);
+          phib[k] = v.c;
+        }
+    }
+    stb(phi_next, bo4, phase_sum_dev(e, phib, hfull));
+  } else {
+    float hv[Q7];
+    pull7<KIND>(g, i, h2, Phi7{}, nonfluid, hv);
+    R phiSum = 0;
+#pragma unroll
+    for (int k = 0; k < Q7; k++) phiSum += (R)hv[k];
+    const R pp = phiSum < 0 ? R(0) : phiSum > 1 ? R(1) : phiSum;
+    stb(phi_next, bo4, (float)pp);
+  }
 }
 template <int KIND>
 __global__ __launch_bounds__(256) void km_phase_stream(const Scene *__restrict__ scenes,
@@ -615,6 +885,170 @@ __global__ __launch_bounds__(256) void k1_phase_stream(const Scene sc, int q) {
 // leaves the state exactly as the reference's step does, and phi(t+1) is
 // written here.  first: the first step of the call (the armed external force
 // applies to its macroscopic update).
+// Macro(t) and either collide(t+1) or the post-stream populations of fluid cell
+// i, from its pulled populations (fp after bounce-back, with their sums pSumF
+// and momentum mx, my, mz; hp) and the gradients of phi(t) at i.  band: the cell
+// lies in the band box (its collide(t+1) runs after the band update).  The 27 +
+// 7 populations to store are handed to stf(k, value, baseline) and
+// sth(k, value, baseline).
+template <bool COLLIDE, int KIND, typename StF, typename StH>
+__device__ __forceinline__ void finish_cell(const Scene &sc, int64_t i, bool band,
+                                            const float *__restrict__ phi_next, float gxo,
+                                            float gyo, float gzo, float lapo, float fp[Q27],
+                                            float hp[Q7], typename Codec<KIND>::R pSumF,
+                                            typename Codec<KIND>::R mx,
+                                            typename Codec<KIND>::R my,
+                                            typename Codec<KIND>::R mz, int use_bf,
+                                            float *__restrict__ phi_out, StF &&stf, StH &&sth) {
+  using C = Codec<KIND>;
+  using R = typename C::R;
+  const Geom &g = sc.g;
+  const Derived &d = sc.d;
+  const R rhoN = pSumF > R(0.01) ? pSumF : R(0.01);
+  R phiSum = 0;
+#pragma unroll
+  for (int k = 0; k < Q7; k++) phiSum += (R)hp[k];
+  const R p = phiSum < 0 ? R(0) : phiSum > 1 ? R(1) : phiSum;
+  const R rhoPhaseNew = R(d.rhoG) + p * R(d.dRho);
+  const R mu = 4 * R(d.beta) * p * (p - 1) * (2 * p - 1) - R(d.kappa) * (R)lapo;
+  const uint32_t bo4 = (uint32_t)i * 4u;
+  const R bxf = use_bf ? (R)ldb(sc.bfx, bo4) : R(0);
+  const R byf = use_bf ? (R)ldb(sc.bfy, bo4) : R(0);
+  const R bzf = use_bf ? (R)ldb(sc.bfz, bo4) : R(0);
+  const R fx = bxf + mu * (R)gxo;
+  const R fy = byf + mu * (R)gyo + rhoPhaseNew * R(d.gravity);
+  const R fz = bzf + mu * (R)gzo;
+  const R invRho = 1 / rhoPhaseNew;
+  R uxn = (mx + R(0.5) * fx) * invRho;
+  R uyn = (my + R(0.5) * fy) * invRho;
+  R uzn = (mz + R(0.5) * fz) * invRho;
+  const R u2 = uxn * uxn + uyn * uyn + uzn * uzn;
+  const R UMAX = R(0.2), UMAX2 = UMAX * UMAX;
+  if (u2 > UMAX2) {
+    const R scl = UMAX / sqrt(u2);
+    uxn *= scl;
+    uyn *= scl;
+    uzn *= scl;
+  }
+  const float rhof = (float)rhoN, phif = (float)p;
+  const float uxf = (float)uxn, uyf = (float)uyn, uzf = (float)uzn;
+  // rho and u are read before the end of the call only inside the band (by
+  // the fresh-cell pass, the dead-cell impulses and the band collide); the
+  // last step of a call stores them everywhere.
+  if (!COLLIDE || band) {
+    stb(sc.rho, bo4, rhof);
+    stb(sc.ux, bo4, uxf);
+    stb(sc.uy, bo4, uyf);
+    stb(sc.uz, bo4, uzf);
+  }
+  if (COLLIDE && !band) {
+    // collide(t+1) with the new macroscopic values (rounded to float as the
+    // reference stores them) and the gradients of phi(t+1).  The post-stream
+    // populations enter as storing and reloading them would give them back
+    // (exact for float32), so a cell collides the same whether or not a call
+    // ends between the stream and the collide.
+    if (KIND != ST_F32) {
+#pragma unroll
+      for (int k = 0; k < Q27; k++) fp[k] = C::rt(fp[k], w27(k));
+#pragma unroll
+      for (int k = 0; k < Q7; k++) hp[k] = C::rt(hp[k], w7(k) * phif);
+    }
+    float gx1, gy1, gz1, lap1;
+    phi_grad<R>(g, phi_next, i, (R)phif, gx1, gy1, gz1, lap1);
+    float hc[Q7];
+    collide_phase_cell<R>(d, (R)phif, (R)uxf, (R)uyf, (R)uzf, (R)gx1, (R)gy1, (R)gz1, hp, hc);
+#pragma unroll
+    for (int k = 0; k < Q7; k++) sth(k, hc[k], w7(k) * phif);
+    float fc[Q27];
+    collide_hydro_cell<R>(d, (R)rhof, (R)phif, (R)lap1, (R)gx1, (R)gy1, (R)gz1, R(0), R(0), R(0),
+                          (R)uxf, (R)uyf, (R)uzf, fp, fc);
+#pragma unroll
+    for (int k = 0; k < Q27; k++) stf(k, fc[k], w27(k));
+  } else {
+    if (!COLLIDE) stb(phi_out, bo4, phif);
+#pragma unroll
+    for (int k = 0; k < Q27; k++) stf(k, fp[k], w27(k));
+#pragma unroll
+    for (int k = 0; k < Q7; k++) sth(k, hp[k], w7(k) * phif);
+  }
+}
+
+// finish_cell in deviation form (16-bit codecs): fp are the collide inputs
+// (f_k - w_k), e the pulled h deviations with their baselines' phi phib, sumD
+// and mx, my, mz the deviation sums.  stf(k, dev) and sth(k, dev) receive the
+// deviations to store (h against w7_k phi(t+1) of the cell).
+template <bool COLLIDE, int KIND, typename StF, typename StH>
+__device__ __forceinline__ void finish_cell_dev(const Scene &sc, int64_t i, bool band,
+                                                const float *__restrict__ phi_next, float gxo,
+                                                float gyo, float gzo, float lapo, float fp[Q27],
+                                                const float e[Q7], const float phib[Q7], float sumD,
+                                                float mx, float my, float mz, int use_bf,
+                                                float *__restrict__ phi_out, StF &&stf,
+                                                StH &&sth) {
+  using C = Codec<KIND>;
+  const Geom &g = sc.g;
+  const Derived &d = sc.d;
+  const float pSum = 1.0f + sumD;
+  const float rhoN = pSum > 0.01f ? pSum : 0.01f;
+  float h[Q7];
+  const float p = phase_sum_dev(e, phib, h);
+  const float rhoPhaseNew = (float)d.rhoG + p * (float)d.dRho;
+  const float mu = 4 * (float)d.beta * p * (p - 1) * (2 * p - 1) - (float)d.kappa * lapo;
+  const uint32_t bo4 = (uint32_t)i * 4u;
+  const float bxf = use_bf ? ldb(sc.bfx, bo4) : 0.0f;
+  const float byf = use_bf ? ldb(sc.bfy, bo4) : 0.0f;
+  const float bzf = use_bf ? ldb(sc.bfz, bo4) : 0.0f;
+  const float fx = bxf + mu * gxo;
+  const float fy = byf + mu * gyo + rhoPhaseNew * (float)d.gravity;
+  const float fz = bzf + mu * gzo;
+  const float invRho = 1 / rhoPhaseNew;
+  float uxn = (mx + 0.5f * fx) * invRho;
+  float uyn = (my + 0.5f * fy) * invRho;
+  float uzn = (mz + 0.5f * fz) * invRho;
+  const float u2 = uxn * uxn + uyn * uyn + uzn * uzn;
+  constexpr float UMAX = 0.2f;
+  if (u2 > UMAX * UMAX) {
+    const float scl = UMAX / sqrtf(u2);
+    uxn *= scl;
+    uyn *= scl;
+    uzn *= scl;
+  }
+  const float rhof = rhoN;
+  if (!COLLIDE || band) {
+    stb(sc.rho, bo4, rhof);
+    stb(sc.ux, bo4, uxn);
+    stb(sc.uy, bo4, uyn);
+    stb(sc.uz, bo4, uzn);
+  }
+  // h against the cell's new phi, the baseline it is stored with.
+  float hd[Q7];
+#pragma unroll
+  for (int k = 0; k < Q7; k++) hd[k] = fmaf(-w7(k), p, h[k]);
+  if (COLLIDE && !band) {
+#pragma unroll
+    for (int k = 0; k < Q7; k++) hd[k] = C::rtv(hd[k]);
+    float gx1, gy1, gz1, lap1;
+    phi_grad<float>(g, phi_next, i, p, gx1, gy1, gz1, lap1);
+    float hc[Q7];
+    collide_phase_dev(d, p, uxn, uyn, uzn, gx1, gy1, gz1, hd, hc);
+#pragma unroll
+    for (int k = 0; k < Q7; k++) sth(k, hc[k]);
+    float fc[Q27];
+    collide_hydro_dev(d, rhof, p, lap1, gx1, gy1, gz1, 0.0f, 0.0f, 0.0f, uxn, uyn, uzn, fp, fc);
+#pragma unroll
+    for (int k = 0; k < Q27; k++) stf(k, fc[k]);
+  } else {
+    if (!COLLIDE) stb(phi_out, bo4, p);
+#pragma unroll
+    for (int k = 0; k < Q27; k++) stf(k, fp[k]);
+#pragma unroll
+    for (int k = 0; k < Q7; k++) sth(k, hd[k]);
+  }
+}
+
+// stream(t) + macro(t) for every fluid cell, then collide(t+1) outside the
+// band; one cell per lane, tiles of 256 cells.  COLLIDE false: the stream-only
+// step that ends a call (phi(t+1) written here, rho/u stored everywhere).
 template <bool COLLIDE, int KIND>
 __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int q, int first) {
   using C = Codec<KIND>;
@@ -655,14 +1089,58 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
     // Solid cells lie inside the band box with a margin of 2, so outside it the
     // only non-fluid cells are the walls on the grid faces.
     const bool near = in_box(box, x, y, z);
-    const bool fluid = interior(g, x, y, z) && (!near || tag[i] == TAG_FLUID);
+    const uint32_t bo4 = (uint32_t)i * 4u;
+    const bool fluid = interior(g, x, y, z) && (!near || ldb(tag, (uint32_t)i) == TAG_FLUID);
     if (!fluid) {
-      if (!COLLIDE) phi_next[i] = phi[i];
+      if (!COLLIDE) stb(phi_next, bo4, ldb(phi, bo4));
+    } else if constexpr (KIND != ST_F32) {
+      const Phi7 v = load_phi7(g, phi, bo4);
+      const float phiOld = v.c;
+      const float rhoPhaseOld = (float)d.rhoG + phiOld * (float)d.dRho;
+      float gxo, gyo, gzo, lapo;
+      phi_grad7<float>(v, phiOld, gxo, gyo, gzo, lapo);
+      uint32_t nonfluid, solid;
+      if (near) tag_masks(g, i, tag, nonfluid, solid);
+      else {
+        nonfluid = wall_mask(g, x, y, z);
+        solid = 0;
+      }
+      const uint32_t bo = (uint32_t)i * (uint32_t)sizeof(T);
+      float raw[Q27];
+#pragma unroll
+      for (int k = 0; k < Q27; k++)
+        raw[k] = C::ldd(f2 + ((int64_t)k * g.S - dir_off(g, k)), bo);
+      float e[Q7], phib[Q7];
+#pragma unroll
+      for (int k = 0; k < Q7; k++) {
+        e[k] = C::ldd(h2 + ((int64_t)k * g.S - dir_off(g, k)), bo);
+        phib[k] = v.src(k);
+      }
+      if (nonfluid) {
+#pragma unroll
+        for (int k = 1; k < Q27; k++)
+          if ((nonfluid >> k) & 1u) raw[k] = C::ldd(f2 + (int64_t)opp27(k) * g.S, bo);
+#pragma unroll
+        for (int k = 1; k < Q7; k++)
+          if ((nonfluid >> k) & 1u) {
+            e[k] = C::ldd(h2 + (int64_t)opp7(k) * g.S, bo);
+            phib[k] = v.c;
+          }
+      }
+      float fp[Q27];
+      float sumD, mx, my, mz;
+      links27_dev<KIND>(g, i, raw, solid, sc.usx, sc.usy, sc.usz, rhoPhaseOld, exchange, x, y, z,
+                        cx, cy, cz, mom, fp, sumD, mx, my, mz);
+      finish_cell_dev<COLLIDE, KIND>(
+          sc, i, near, phi_next, gxo, gyo, gzo, lapo, fp, e, phib, sumD, mx, my, mz, use_bf,
+          phi_next, [&](int k, float dv) { C::std_(fo + (int64_t)k * g.S, bo, dv); },
+          [&](int k, float dv) { C::std_(ho + (int64_t)k * g.S, bo, dv); });
     } else {
-      const R phiOld = phi[i];
+      const Phi7 v = load_phi7(g, phi, bo4);
+      const R phiOld = v.c;
       const R rhoPhaseOld = R(d.rhoG) + phiOld * R(d.dRho);
       float gxo, gyo, gzo, lapo;
-      phi_grad<R>(g, phi, i, phiOld, gxo, gyo, gzo, lapo);
+      phi_grad7<R>(v, phiOld, gxo, gyo, gzo, lapo);
       uint32_t nonfluid, solid;
       if (near) tag_masks(g, i, tag, nonfluid, solid);
       else {
@@ -673,82 +1151,13 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
       R pSumF, mx, my, mz;
       pull27<KIND>(g, i, f2, nonfluid, solid, sc.usx, sc.usy, sc.usz, rhoPhaseOld, exchange, x, y,
                    z, cx, cy, cz, mom, fp, pSumF, mx, my, mz);
-      const R rhoN = pSumF > R(0.01) ? pSumF : R(0.01);
       float hp[Q7];
-      pull7<KIND>(g, i, h2, phi, nonfluid, hp);
-      R phiSum = 0;
-#pragma unroll
-      for (int k = 0; k < Q7; k++) phiSum += (R)hp[k];
-      const R p = phiSum < 0 ? R(0) : phiSum > 1 ? R(1) : phiSum;
-      const R rhoPhaseNew = R(d.rhoG) + p * R(d.dRho);
-      const R mu = 4 * R(d.beta) * p * (p - 1) * (2 * p - 1) - R(d.kappa) * (R)lapo;
-      const R bxf = use_bf ? (R)sc.bfx[i] : R(0);
-      const R byf = use_bf ? (R)sc.bfy[i] : R(0);
-      const R bzf = use_bf ? (R)sc.bfz[i] : R(0);
-      const R fx = bxf + mu * (R)gxo;
-      const R fy = byf + mu * (R)gyo + rhoPhaseNew * R(d.gravity);
-      const R fz = bzf + mu * (R)gzo;
-      const R invRho = 1 / rhoPhaseNew;
-      R uxn = (mx + R(0.5) * fx) * invRho;
-      R uyn = (my + R(0.5) * fy) * invRho;
-      R uzn = (mz + R(0.5) * fz) * invRho;
-      const R u2 = uxn * uxn + uyn * uyn + uzn * uzn;
-      const R UMAX = R(0.2), UMAX2 = UMAX * UMAX;
-      if (u2 > UMAX2) {
-        const R scl = UMAX / sqrt(u2);
-        uxn *= scl;
-        uyn *= scl;
-        uzn *= scl;
-      }
-      const float rhof = (float)rhoN, phif = (float)p;
-      const float uxf = (float)uxn, uyf = (float)uyn, uzf = (float)uzn;
-      const bool band = near;
-      // rho and u are read before the end of the call only inside the band (by
-      // the fresh-cell pass, the dead-cell impulses and the band collide); the
-      // last step of a call stores them everywhere.
-      if (!COLLIDE || band) {
-        sc.rho[i] = rhof;
-        sc.ux[i] = uxf;
-        sc.uy[i] = uyf;
-        sc.uz[i] = uzf;
-      }
-      bool store_stream = true;
-      if (COLLIDE) {
-        if (!band) {
-          store_stream = false;
-          // collide(t+1) with the new macroscopic values (rounded to float as
-          // the reference stores them) and the gradients of phi(t+1).  The
-          // post-stream populations enter as storing and reloading them would
-          // give them back (exact for float32), so a cell collides the same
-          // whether or not a call ends between the stream and the collide.
-          if (KIND != ST_F32) {
-#pragma unroll
-            for (int k = 0; k < Q27; k++) fp[k] = C::rt(fp[k], w27(k));
-#pragma unroll
-            for (int k = 0; k < Q7; k++) hp[k] = C::rt(hp[k], w7(k) * phif);
-          }
-          float gx1, gy1, gz1, lap1;
-          phi_grad<R>(g, phi_next, i, (R)phif, gx1, gy1, gz1, lap1);
-          float hc[Q7];
-          collide_phase_cell<R>(d, (R)phif, (R)uxf, (R)uyf, (R)uzf, (R)gx1, (R)gy1, (R)gz1, hp,
-                                hc);
-#pragma unroll
-          for (int k = 0; k < Q7; k++) C::st(ho, k * g.S + i, hc[k], w7(k) * phif);
-          float fc[Q27];
-          collide_hydro_cell<R>(d, (R)rhof, (R)phif, (R)lap1, (R)gx1, (R)gy1, (R)gz1, R(0), R(0),
-                                R(0), (R)uxf, (R)uyf, (R)uzf, fp, fc);
-#pragma unroll
-          for (int k = 0; k < Q27; k++) C::st(fo, k * g.S + i, fc[k], w27(k));
-        }
-      } else {
-        phi_next[i] = phif;
-      }
-      if (store_stream) {
-#pragma unroll
-        for (int k = 0; k < Q27; k++) C::st(fo, k * g.S + i, fp[k], w27(k));
-#pragma unroll
-        for (int k = 0; k < Q7; k++) C::st(ho, k * g.S + i, hp[k], w7(k) * phif);
-      }
+      pull7<KIND>(g, i, h2, v, nonfluid, hp);
+      const uint32_t bo = (uint32_t)i * (uint32_t)sizeof(T);
+      finish_cell<COLLIDE, KIND>(
+          sc, i, near, phi_next, gxo, gyo, gzo, lapo, fp, hp, pSumF, mx, my, mz, use_bf, phi_next,
+          [&](int k, float val, float base) { C::sto(fo + (int64_t)k * g.S, bo, val, base); },
+          [&](int k, float val, float base) { C::sto(ho + (int64_t)k * g.S, bo, val, base); });
     }
   }
   if (exchange) tile_momentum(mom, sc.partials, tile);
@@ -764,6 +1173,283 @@ template <bool COLLIDE, int KIND>
 __global__ __launch_bounds__(256) void k1_stream_collide(const Scene sc, int q, int first) {
   stream_collide_body<COLLIDE, KIND>(sc, blockIdx.x, q, first);
 }
+// ---------------------------------------------------------------- stream + collide, two cells per lane
+
+// The update of stream_collide_body for the 16-bit codecs with each lane owning
+// two consecutive cells (i0, i0 + 1), i0 even: every population plane is read
+// and written in naturally aligned 4-byte pairs instead of 2-byte elements,
+// twice the bytes per memory instruction (the bare 34-plane pull pattern on
+// MI355X: 5.2 against 3.8 TB/s).  A tile of 256 cells is one workgroup of two
+// waves, so tiles, their order and the momentum partials are the one-cell
+// kernel's.
+//
+// The source pair of a direction, (i0 - off, i0 + 1 - off), is an aligned pair
+// when off is even.  When off is odd it is the upper element of the aligned pair
+// at i0 - off - 1 and the lower element of the next lane's; the wave's last lane
+// loads that pair itself.  The parity of off follows from the parities of nx and
+// nx ny (the parity class of the scene), so it is known per direction at
+// compile time and all lanes take the same path.
+//
+// The two cells are updated one after the other.  Their pulled populations,
+// and then their updated ones, stay packed in one register per direction, so
+// the registers hold the working set of one cell at a time.  A pair holding a
+// fluid cell is stored whole: the element of a non-fluid cell receives a zero
+// deviation (populations of non-fluid cells are never read).
+
+// 0: nx even; 1: nx odd, ny even; 2: nx and ny odd.
+__host__ __device__ inline int parity_class(int nx, int ny) {
+  return (nx & 1) == 0 ? 0 : ((ny & 1) == 0 ? 1 : 2);
+}
+// Parity of the pull offset ez nx ny + ey nx + ex of direction K in class PC
+// (D3Q7 directions are D3Q27 directions 0..6).
+template <int PC, int K>
+struct PullOdd {
+  static constexpr bool v =
+      ((EX27[K] + (PC != 0 ? EY27[K] : 0) + (PC == 2 ? EZ27[K] : 0)) & 1) != 0;
+};
+
+// f(std::integral_constant<int, K>) for K = B .. E - 1.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// Buffer resource over a population buffer of `bytes` bytes (raw addressing,
+// no stride; word 3 as CDNA expects for untyped loads).  A load outside
+// [0, bytes) returns 0 and a store there is dropped, which is how the pulls
+// below stay in bounds without clamping: only lanes without a fluid cell
+// address outside their buffer, and their values are never used.  Offsets are
+// 32-bit byte offsets, so the buffers must be smaller than 4 GiB minus the
+// largest pull offset (checked on the host).
+constexpr int kRsrcWord3 = 0x00020000;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pop_rsrc(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, kRsrcWord3);
+}
+__device__ __forceinline__ uint32_t ld_pair(__amdgpu_buffer_rsrc_t r, uint32_t e) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, (int)(2u * e), 0, 0);
+}
+__device__ __forceinline__ void st_pair(__amdgpu_buffer_rsrc_t r, uint32_t e, uint32_t v) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)(2u * e), 0, 0);
+}
+
+// Elements p0 + i0 - off and p0 + i0 + 1 - off of a buffer of 16-bit elements
+// (p0: the first element of the plane), packed, the first in the low half.
+// Element indices wrap modulo 2^32 (a source before the buffer becomes an
+// offset past its end).  last: wbase + 127, the second cell of the wave's last
+// lane; for odd off every lane loads that lane's upper pair from the same
+// address (one request).
+template <bool ODD>
+__device__ __forceinline__ uint32_t pull_pair(__amdgpu_buffer_rsrc_t r, uint32_t p0, uint32_t i0,
+                                              uint32_t off, uint32_t last, int lane) {
+  if constexpr (!ODD) {
+    return ld_pair(r, p0 + i0 - off);
+  } else {
+    const uint32_t w = ld_pair(r, p0 + i0 - off - 1u);
+    const uint32_t edge = ld_pair(r, p0 + last - off);
+    uint32_t nxt = (uint32_t)__shfl_down((int)w, 1, 64);
+    nxt = lane == 63 ? edge : nxt;
+    return (w >> 16) | (nxt << 16);
+  }
+}
+
+// red: 6 x 256 doubles of workgroup memory for the tile's link momentum.
+template <bool COLLIDE, int KIND, int PC>
+__device__ __forceinline__ void stream_collide_pair_body(const Scene &sc, int tile, int q,
+                                                         int first, double (*red)[256]) {
+  using C = Codec<KIND>;
+  using R = typename C::R;
+  static_assert(KIND != ST_F32, "16-bit codecs only");
+  const Geom &g = sc.g;
+  const Derived &d = sc.d;
+  const int64_t S = g.S;
+  const uint32_t S32 = (uint32_t)S;
+  const int lane = threadIdx.x;
+  const int64_t wbase = ((int64_t)tile << kTileShift) + 128 * (int)threadIdx.y;
+  const int64_t i0 = wbase + 2 * lane;
+  const uint32_t i0u = (uint32_t)i0, last = (uint32_t)(wbase + 127);
+  q &= 1;
+  const __amdgpu_buffer_rsrc_t f2 = pop_rsrc(par(sc.f, q), (uint32_t)(Q27 * 2 * S));
+  const __amdgpu_buffer_rsrc_t h2 = pop_rsrc(par(sc.h, q), (uint32_t)(Q7 * 2 * S));
+  const __amdgpu_buffer_rsrc_t fo = pop_rsrc(par(sc.f, q ^ 1), (uint32_t)(Q27 * 2 * S));
+  const __amdgpu_buffer_rsrc_t ho = pop_rsrc(par(sc.h, q ^ 1), (uint32_t)(Q7 * 2 * S));
+  const float *__restrict__ phi = par(sc.phi, q);
+  float *__restrict__ phi_next = par(sc.phi, q ^ 1);
+  const uint8_t *__restrict__ tag = sc.tag;
+  const Ctrl *__restrict__ ctrl = sc.ctrl;
+  int box[6];
+#pragma unroll
+  for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
+  const bool exchange = sc.has_solid != 0 && tile_in_box_rows(sc.tm, box, tile);
+  const int use_bf = first && sc.use_bf;
+  double cx = 0, cy = 0, cz = 0;
+  if (exchange) {
+    cx = ctrl->sphere.cx;
+    cy = ctrl->sphere.cy;
+    cz = ctrl->sphere.cz;
+  }
+  // Both cells: coordinates, fluid flags and source masks (neighbour tags
+  // inside the band box, the walls elsewhere).
+  int xs[2], ys[2], zs[2];
+  bool fl[2], nr[2];
+  uint32_t nf[2], so[2];
+#pragma unroll
+  for (int J = 0; J < 2; J++) {
+    const int64_t i = i0 + J;
+    cell_xyz(g.nx, g.nxny, i, xs[J], ys[J], zs[J]);
+    nr[J] = in_box(box, xs[J], ys[J], zs[J]);
+    fl[J] = i < g.n && interior(g, xs[J], ys[J], zs[J]) && (!nr[J] || tag[i] == TAG_FLUID);
+    nf[J] = so[J] = 0;
+    if (fl[J]) {
+      if (nr[J]) tag_masks(g, i, tag, nf[J], so[J]);
+      else nf[J] = wall_mask(g, xs[J], ys[J], zs[J]);
+    }
+  }
+  // Pulled pairs.
+  uint32_t fr[Q27], hr[Q7];
+  static_for<0, Q27>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    fr[k] = pull_pair<PullOdd<PC, k>::v>(f2, k * S32, i0u, (uint32_t)dir_off(g, k), last, lane);
+  });
+  static_for<0, Q7>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    hr[k] = pull_pair<PullOdd<PC, k>::v>(h2, k * S32, i0u, (uint32_t)dir_off(g, k), last, lane);
+  });
+  // Halfway bounce-back: the own opposite population where a source is not
+  // fluid, requested for every such direction before any is merged.
+  const uint32_t nfa = nf[0] | nf[1];
+  if (nfa) {
+    uint32_t ob[Q27], oh[Q7];
+#pragma unroll
+    for (int k = 1; k < Q27; k++)
+      ob[k] = ((nfa >> k) & 1u) ? ld_pair(f2, opp27(k) * S32 + i0u) : 0u;
+#pragma unroll
+    for (int k = 1; k < Q7; k++)
+      oh[k] = ((nfa >> k) & 1u) ? ld_pair(h2, opp7(k) * S32 + i0u) : 0u;
+#pragma unroll
+    for (int k = 1; k < Q27; k++) {
+      const uint32_t bm = (((nf[0] >> k) & 1u) ? 0x0000ffffu : 0u) |
+                          (((nf[1] >> k) & 1u) ? 0xffff0000u : 0u);
+      fr[k] = (fr[k] & ~bm) | (ob[k] & bm);
+    }
+#pragma unroll
+    for (int k = 1; k < Q7; k++) {
+      const uint32_t bm = (((nf[0] >> k) & 1u) ? 0x0000ffffu : 0u) |
+                          (((nf[1] >> k) & 1u) ? 0xffff0000u : 0u);
+      hr[k] = (hr[k] & ~bm) | (oh[k] & bm);
+    }
+  }
+  // Link momentum per cell, by cell of the tile (component-major), in red;
+  // reduced below in the one-cell kernel's order.
+#pragma unroll
+  for (int J = 0; J < 2; J++) {
+    const int64_t i = i0 + J;
+      double mom[6] = {0, 0, 0, 0, 0, 0};
+    if (fl[J]) {
+      const int sh = 16 * J;
+      const uint32_t keep = ~(0xffffu << sh);
+      const uint32_t nonfluid = nf[J], solid = so[J];
+      const int x = xs[J], y = ys[J], z = zs[J];
+      const Phi7 v = load_phi7(g, phi, (uint32_t)i * 4u);
+      const float phiOld = v.c;
+      const float rhoPhaseOld = (float)d.rhoG + phiOld * (float)d.dRho;
+      float gxo, gyo, gzo, lapo;
+      phi_grad7<float>(v, phiOld, gxo, gyo, gzo, lapo);
+      float raw[Q27];
+#pragma unroll
+      for (int k = 0; k < Q27; k++) raw[k] = C::dv((uint16_t)(fr[k] >> sh));
+      float fp[Q27];
+      float sumD, mx, my, mz;
+      links27_dev<KIND>(g, i, raw, solid, sc.usx, sc.usy, sc.usz, rhoPhaseOld, exchange, x, y, z,
+                        cx, cy, cz, mom, fp, sumD, mx, my, mz);
+      // A pulled h has the phi of its source as its baseline, a bounced one the
+      // phi of the cell.
+      float e[Q7], phib[Q7];
+#pragma unroll
+      for (int k = 0; k < Q7; k++) {
+        e[k] = C::dv((uint16_t)(hr[k] >> sh));
+        phib[k] = ((nonfluid >> k) & 1u) ? v.c : v.src(k);
+      }
+      finish_cell_dev<COLLIDE, KIND>(
+          sc, i, nr[J], phi_next, gxo, gyo, gzo, lapo, fp, e, phib, sumD, mx, my, mz, use_bf,
+          phi_next,
+          [&](int k, float dv) { fr[k] = (fr[k] & keep) | ((uint32_t)C::ev(dv) << sh); },
+          [&](int k, float dv) { hr[k] = (hr[k] & keep) | ((uint32_t)C::ev(dv) << sh); });
+    } else if (!COLLIDE && i < g.n) {
+      phi_next[i] = phi[i];
+    }
+    if (exchange) {
+      const int t = (int)(i - ((int64_t)tile << kTileShift));
+#pragma unroll
+      for (int c = 0; c < 6; c++) red[c][t] = mom[c];
+    }
+  }
+  if (fl[0] || fl[1]) {
+    if (!(fl[0] && fl[1])) {
+      // The element of the non-fluid cell gets a zero deviation (its pulled
+      // value may come from outside the grid).
+      const uint32_t m = fl[0] ? 0x0000ffffu : 0xffff0000u;
+#pragma unroll
+      for (int k = 0; k < Q27; k++) fr[k] &= m;
+#pragma unroll
+      for (int k = 0; k < Q7; k++) hr[k] &= m;
+    }
+#pragma unroll
+    for (int k = 0; k < Q27; k++) st_pair(fo, k * S32 + i0u, fr[k]);
+#pragma unroll
+    for (int k = 0; k < Q7; k++) st_pair(ho, k * S32 + i0u, hr[k]);
+  }
+  if (exchange) {
+    // The one-cell kernel sums each wave of 64 cells by an xor butterfly with
+    // offsets 32 .. 1 (every lane ends with the value lane 0 computes: at
+    // offset o, cell j < o adds cell j + o to itself), then the four waves in
+    // order.  The same tree, level by level, over the cells of each 64-cell
+    // group gives the same sums bit for bit.
+    const int tid = (int)threadIdx.y * 64 + lane;
+    __syncthreads();
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      for (int w = tid; w < 24 * o; w += 128) {
+        const int c = w / (4 * o), r = w - c * (4 * o), grp = r / o, j = r - grp * o;
+        double *row = red[c] + grp * 64;
+        row[j] = row[j] + row[j + o];
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+#pragma unroll
+      for (int c = 0; c < 6; c++)
+        sc.partials[tile * 6 + c] = ((red[c][0] + red[c][64]) + red[c][128]) + red[c][192];
+    }
+  }
+}
+
+// Workgroups of 64 x 2 threads, one tile each.
+constexpr int kPairWaves = 2;
+template <bool COLLIDE, int KIND>
+__device__ __forceinline__ void stream_collide_pair(const Scene &sc, int lb, int q, int first) {
+  int tile;
+  if (!map_tile(sc.tm.T, sc.tm.T8, lb, tile)) return;
+  __shared__ double red[6][256];
+  const int pc = parity_class(sc.g.nx, sc.g.ny);
+  if (pc == 0) stream_collide_pair_body<COLLIDE, KIND, 0>(sc, tile, q, first, red);
+  else if (pc == 1) stream_collide_pair_body<COLLIDE, KIND, 1>(sc, tile, q, first, red);
+  else stream_collide_pair_body<COLLIDE, KIND, 2>(sc, tile, q, first, red);
+}
+template <bool COLLIDE, int KIND>
+__global__ __launch_bounds__(128) void km_stream_collide2(const Scene *__restrict__ scenes,
+                                                          const int *__restrict__ off, int nscenes,
+                                                          int q, int first) {
+  const int si = find_scene(off, nscenes, blockIdx.x);
+  stream_collide_pair<COLLIDE, KIND>(scenes[si], blockIdx.x - off[si], q, first);
+}
+template <bool COLLIDE, int KIND>
+__global__ __launch_bounds__(128) void k1_stream_collide2(const Scene sc, int q, int first) {
+  stream_collide_pair<COLLIDE, KIND>(sc, blockIdx.x, q, first);
+}
+
 // ---------------------------------------------------------------- storage conversion
 
 // Float32 populations (the reference's f and h) to a 16-bit codec, and back.

@@ -27,6 +27,47 @@ struct Timer {
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
 };
 
+// Executable graphs of captured launch sequences, keyed by the bytes of every
+// launch argument and grid size they were captured with: an entry is replayed
+// only for launches identical to the captured ones.  Least recently used
+// entries are dropped beyond kMax.
+struct GraphCache {
+  static constexpr int kMax = 8;
+  struct Entry {
+    std::vector<char> key;
+    hipGraphExec_t exec;
+    uint64_t used;
+  };
+  std::vector<Entry> entries;
+  uint64_t clock = 0;
+  // Steps per graph (even; 0: no graphs, every kernel launched on its own).
+  int steps = 2;
+  int64_t captures = 0, replays = 0;
+  ~GraphCache() { clear(); }
+  void clear() {
+    for (auto &e : entries) hipGraphExecDestroy(e.exec);
+    entries.clear();
+  }
+  hipGraphExec_t find(const std::vector<char> &key) {
+    for (auto &e : entries)
+      if (e.key == key) {
+        e.used = ++clock;
+        return e.exec;
+      }
+    return nullptr;
+  }
+  void put(std::vector<char> key, hipGraphExec_t exec) {
+    if ((int)entries.size() >= kMax) {
+      size_t lru = 0;
+      for (size_t j = 1; j < entries.size(); j++)
+        if (entries[j].used < entries[lru].used) lru = j;
+      hipGraphExecDestroy(entries[lru].exec);
+      entries.erase(entries.begin() + lru);
+    }
+    entries.push_back({std::move(key), exec, ++clock});
+  }
+};
+
 struct lbm_sim {
   int nx, ny, nz;
   Geom g;
@@ -85,6 +126,8 @@ struct lbm_sim {
   void *cf[2] = {nullptr, nullptr}, *ch[2] = {nullptr, nullptr};
   // Timing.
   Timer timer;
+  GraphCache graphs;  // steady steps of the fused call (lbm_step)
+  int lanes = 2;      // cells per lane of the 16-bit stream kernel (1 or 2)
   std::string err;
 };
 
@@ -133,6 +176,7 @@ static dim3 cell_grid(const lbm_sim *s) {
   return dim3((s->nx + 63) / 64, (s->ny + 3) / 4, s->nz);
 }
 static const dim3 kCellBlock(64, 4, 1);
+static const dim3 kPairBlock(64, 2, 1);  // a tile of 256 cells, two per lane
 
 static float *F(lbm_sim *s) { return s->fbuf[s->fcur]; }
 static float *F2(lbm_sim *s) { return s->fbuf[1 - s->fcur]; }
@@ -363,6 +407,7 @@ extern "C" void lbm_destroy(lbm_sim *s) {
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (s->ctrl_host) hipHostFree(s->ctrl_host);
+  s->graphs.clear();
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;
 }
@@ -994,6 +1039,32 @@ extern "C" int lbm_kernel_time(lbm_sim *s, const char *name, double *total_ms, i
   int rc = sync(s);
   if (rc) return rc;
   timer_read(s->timer, name, total_ms, launches);
+  return LBM_OK;
+}
+
+static int graph_steps_ok(int steps) { return steps == 0 || (steps >= 2 && steps <= 64 && !(steps & 1)); }
+
+extern "C" int lbm_set_graph_steps(lbm_sim *s, int steps) {
+  CHECK_SIM(s);
+  if (!graph_steps_ok(steps)) return fail(s, LBM_ERR_ARG, "graph steps must be 0 or even in 2..64");
+  int rc = sync(s);
+  if (rc) return rc;
+  s->graphs.clear();
+  s->graphs.steps = steps;
+  return LBM_OK;
+}
+
+extern "C" int lbm_set_stream_lanes(lbm_sim *s, int lanes) {
+  CHECK_SIM(s);
+  if (lanes != 1 && lanes != 2) return fail(s, LBM_ERR_ARG, "cells per lane must be 1 or 2");
+  s->lanes = lanes;
+  return LBM_OK;
+}
+
+extern "C" int lbm_graph_counts(const lbm_sim *s, int64_t *captures, int64_t *replays) {
+  CHECK_SIM(s);
+  if (captures) *captures = s->graphs.captures;
+  if (replays) *replays = s->graphs.replays;
   return LBM_OK;
 }
 

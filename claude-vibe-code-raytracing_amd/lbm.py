@@ -153,6 +153,9 @@ def lib():
         "lbm_set_timing": (C.c_int, [P, C.c_int]),
         "lbm_kernel_time": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double),
                                       C.POINTER(C.c_int64)]),
+        "lbm_set_graph_steps": (C.c_int, [P, C.c_int]),
+        "lbm_graph_counts": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "lbm_set_stream_lanes": (C.c_int, [P, C.c_int]),
         "lbm_version": (C.c_char_p, []),
         "lbm_probe_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
         "lbm_state_bytes": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -175,6 +178,9 @@ def lib():
         "lbm_batch_set_timing": (C.c_int, [P, C.c_int]),
         "lbm_batch_kernel_time": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double),
                                             C.POINTER(C.c_int64)]),
+        "lbm_batch_set_graph_steps": (C.c_int, [P, C.c_int]),
+        "lbm_batch_graph_counts": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "lbm_batch_set_stream_lanes": (C.c_int, [P, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -410,6 +416,20 @@ class Sim:
                  "lbm_kernel_time")
         return ms.value, cnt.value
 
+    def set_graph_steps(self, steps):
+        """Steps per captured launch graph of coupled(); 0 launches kernels one by one."""
+        self._ck(self.L.lbm_set_graph_steps(self.h, int(steps)), "lbm_set_graph_steps")
+
+    def set_stream_lanes(self, cells_per_lane):
+        """Cells per lane (1 or 2) of the 16-bit stream kernel."""
+        self._ck(self.L.lbm_set_stream_lanes(self.h, int(cells_per_lane)), "lbm_set_stream_lanes")
+
+    def graph_counts(self):
+        """(captures, replays) of launch graphs since the handle was created."""
+        a, b = C.c_int64(0), C.c_int64(0)
+        self._ck(self.L.lbm_graph_counts(self.h, C.byref(a), C.byref(b)), "lbm_graph_counts")
+        return a.value, b.value
+
 
 class Batch:
     """Many simulations stepped by the same launches (struct lbm_batch)."""
@@ -462,6 +482,19 @@ class Batch:
         self._ck(self.L.lbm_batch_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(cnt)),
                  "lbm_batch_kernel_time")
         return ms.value, cnt.value
+
+    def set_graph_steps(self, steps):
+        self._ck(self.L.lbm_batch_set_graph_steps(self.h, int(steps)), "lbm_batch_set_graph_steps")
+
+    def set_stream_lanes(self, cells_per_lane):
+        self._ck(self.L.lbm_batch_set_stream_lanes(self.h, int(cells_per_lane)),
+                 "lbm_batch_set_stream_lanes")
+
+    def graph_counts(self):
+        a, b = C.c_int64(0), C.c_int64(0)
+        self._ck(self.L.lbm_batch_graph_counts(self.h, C.byref(a), C.byref(b)),
+                 "lbm_batch_graph_counts")
+        return a.value, b.value
 
 
 _live_servers = weakref.WeakSet()

@@ -225,6 +225,13 @@ int lbm_batch_step(lbm_batch *batch, int nsteps);
 /* Per-kernel HIP event timing of lbm_batch_step, as lbm_set_timing. */
 int lbm_batch_set_timing(lbm_batch *batch, int on);
 int lbm_batch_kernel_time(lbm_batch *batch, const char *name, double *total_ms, int64_t *launches);
+/* Launch graphs of lbm_batch_step, as lbm_set_graph_steps / lbm_graph_counts.
+ * A set's graph reads the scene table from device memory, so it is replayed
+ * for any members whose launch grids match the captured ones. */
+int lbm_batch_set_graph_steps(lbm_batch *batch, int steps);
+int lbm_batch_graph_counts(const lbm_batch *batch, int64_t *captures, int64_t *replays);
+/* Cells per lane of the batch's 16-bit stream kernel, as lbm_set_stream_lanes. */
+int lbm_batch_set_stream_lanes(lbm_batch *batch, int cells_per_lane);
 
 /* ------------------------------------------------------------------ serving */
 /* A request loop that keeps many scenes resident and steps them with
@@ -360,6 +367,24 @@ int lbm_set_timing(lbm_sim *sim, int on);
  * Names: "stream_collide", "phase_stream", "band", "body", "collide_all",
  * "stream_macro". */
 int lbm_kernel_time(lbm_sim *sim, const char *name, double *total_ms, int64_t *launches);
+
+/* Launch graphs of lbm_step.  No reference counterpart (the reference calls
+ * its passes from JavaScript, src/sim.js:202-210).  Every step of a call but
+ * the first and the last launches the same kernels with the same arguments
+ * except the step parity; runs of `steps` such steps (even, >= 2) are captured
+ * once into a HIP graph and replayed, here and in later calls with identical
+ * launch arguments.  steps = 0 launches every kernel on its own.  Results are
+ * the same either way, bit for bit.  Timing (lbm_set_timing) launches kernels
+ * on their own.  Default 2. */
+int lbm_set_graph_steps(lbm_sim *sim, int steps);
+/* Graph captures and graph replays since the handle was created. */
+int lbm_graph_counts(const lbm_sim *sim, int64_t *captures, int64_t *replays);
+/* Cells per lane of the fused stream kernel with 16-bit storage: 2 (default)
+ * reads and writes each population plane in aligned 4-byte pairs of cells, 1
+ * in single 2-byte elements.  No reference counterpart.  Both give the same
+ * results bit for bit at every fluid cell (populations of non-fluid cells are
+ * unspecified); float32 storage always runs one cell per lane. */
+int lbm_set_stream_lanes(lbm_sim *sim, int cells_per_lane);
 
 /* Measured device bandwidth of a plain copy: copies `bytes` (rounded down to 16)
  * between two device buffers `reps` times with one float4 per thread and
