@@ -25,6 +25,18 @@ struct Derived {
   double invTauPhi, sharpC, sharpFac;
 };
 
+// The same constants rounded to float, for the FP32 arithmetic of the 16-bit
+// codecs (half the scalar registers of the doubles, no conversions per use).
+struct DerivedF {
+  float invTau, beta, kappa, dRho, A, rhoG, gravity;
+  float invTauPhi, sharpC, sharpFac;
+};
+inline DerivedF derive_f(const Derived &d) {
+  return DerivedF{(float)d.invTau, (float)d.beta,      (float)d.kappa,  (float)d.dRho,
+                  (float)d.A,      (float)d.rhoG,      (float)d.gravity, (float)d.invTauPhi,
+                  (float)d.sharpC, (float)d.sharpFac};
+}
+
 // Device control block: list counters, error flags, the device copy of the
 // sphere and the band boxes of the fused step.
 struct Ctrl {
