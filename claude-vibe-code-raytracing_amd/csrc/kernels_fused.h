@@ -434,9 +434,8 @@ __device__ __forceinline__ void links27(const Geom &g, int64_t i, const float ra
 }
 
 // Pull stream of the 27 populations with halfway bounce-back, given the source
-// masks: every population is pulled from its source (inside the grid for an
-// interior cell), all 27 loads in flight together; lanes with a non-fluid
-// source then take their own opposite populations instead.
+// masks.  Every load address is chosen without branching (own opposite
+// population at a non-fluid source), so the 27 loads are in flight together.
 template <int KIND>
 __device__ __forceinline__ void pull27(const Geom &g, int64_t i,
                                        const typename Codec<KIND>::T *__restrict__ f2,
@@ -451,19 +450,12 @@ __device__ __forceinline__ void pull27(const Geom &g, int64_t i,
                                        typename Codec<KIND>::R &mx, typename Codec<KIND>::R &my,
                                        typename Codec<KIND>::R &mz) {
   using C = Codec<KIND>;
-  using T = typename C::T;
-  const uint32_t bo = (uint32_t)i * (uint32_t)sizeof(T);
   float raw[Q27];
 #pragma unroll
   for (int k = 0; k < Q27; k++) {
-    const int64_t off = (int64_t)EZ27[k] * g.nxny + (int64_t)EY27[k] * g.nx + EX27[k];
-    raw[k] = C::ldo(f2 + ((int64_t)k * g.S - off), bo, w27(k));
-  }
-  if (nonfluid) {
-#pragma unroll
-    for (int k = 1; k < Q27; k++)
-      if ((nonfluid >> k) & 1u)
-        raw[k] = C::ldo(f2 + (int64_t)opp27(k) * g.S, bo, w27(k));  // w27(opp(k)) == w27(k)
+    const int64_t s = i - ((int64_t)EZ27[k] * g.nxny + (int64_t)EY27[k] * g.nx + EX27[k]);
+    const int64_t a = ((nonfluid >> k) & 1u) ? opp27(k) * g.S + i : k * g.S + s;
+    raw[k] = C::ld(f2, a, w27(k));  // w27(opp(k)) == w27(k)
   }
   links27<typename C::R>(g, i, raw, solid, usx, usy, usz, rhoPhaseOld, exchange, x, y, z, cx, cy,
                          cz, mom, fp, pSumF, mx, my, mz);
@@ -540,18 +532,12 @@ __device__ __forceinline__ void pull7(const Geom &g, int64_t i,
                                       const typename Codec<KIND>::T *__restrict__ h2,
                                       const Phi7 &v, uint32_t nonfluid, float hp[Q7]) {
   using C = Codec<KIND>;
-  using T = typename C::T;
-  const uint32_t bo = (uint32_t)i * (uint32_t)sizeof(T);
 #pragma unroll
   for (int k = 0; k < Q7; k++) {
-    const int64_t off = (int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k];
-    hp[k] = C::ldo(h2 + ((int64_t)k * g.S - off), bo, KIND != ST_F32 ? w7(k) * v.src(k) : 0.0f);
-  }
-  if (nonfluid & 0x7eu) {
-#pragma unroll
-    for (int k = 1; k < Q7; k++)
-      if ((nonfluid >> k) & 1u)
-        hp[k] = C::ldo(h2 + (int64_t)opp7(k) * g.S, bo, KIND != ST_F32 ? w7(k) * v.c : 0.0f);
+    const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
+    const bool bounce = (nonfluid >> k) & 1u;
+    const int64_t a = bounce ? opp7(k) * g.S + i : k * g.S + s;
+    hp[k] = C::ld(h2, a, KIND != ST_F32 ? w7(k) * (bounce ? v.c : v.src(k)) : 0.0f);
   }
 }
 
