@@ -59,6 +59,10 @@ def measured_traffic(n, storage):
         return None
     return t["traffic_bytes"] if t.get("cells") == n ** 3 else None
 STORAGE_NAME = {"f32": "float32", "f16": "fp16-deviation", "bf16": "bf16-deviation"}
+# The dominant kernel per storage: float32 runs one cell per lane, the 16-bit
+# codecs two (aligned 4-byte pairs).
+STREAM_KERNEL = {"f32": "k1_stream_collide", "f16": "k1_stream_collide2",
+                 "bf16": "k1_stream_collide2"}
 ARITH = {"f32": "f64", "f16": "f32", "bf16": "f32"}
 METRIC = "coupled MLUPS + %HBM peak at 256^3; served scenes MLUPS, p99 frame latency"
 
@@ -122,10 +126,18 @@ def bench_batch(args, copy_gbps):
     print(f"[bench] {len(scenes)} scenes, {cells} cells; warmup {args.warmup}", file=sys.stderr,
           flush=True)
     batch.step(args.warmup)
-    batch.set_timing(True)
+    # Launch shapes from the autotuner (its trial steps are ordinary steps).
+    tune = batch.autotune(args.tune_steps) if args.tune_steps > 0 else None
+    # The timed call launches as in production (steady steps replayed from a
+    # captured graph); lbm_batch_step returns after its stream has drained.
     t0 = time.perf_counter()
     batch.step(args.steps)
     dt = time.perf_counter() - t0
+    # Per-kernel times from a separate call with HIP events around every launch
+    # (kernels launched one by one).
+    ksteps = min(args.steps, 100)
+    batch.set_timing(True)
+    batch.step(ksteps)
     kt = {k: batch.kernel_time(k) for k in ("stream_collide", "phase_stream", "stream_macro",
                                            "body", "band", "collide_all")}
     batch.set_timing(False)
@@ -148,13 +160,16 @@ def bench_batch(args, copy_gbps):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None, "kernel": "km_stream_collide (all scenes per launch)",
-                     "bytes_per_lu": kb, "avg_ms": round(avg_s * 1e3, 4)},
+                     "traffic": None, "kernel": STREAM_KERNEL[args.storage].replace("k1_", "km_")
+                     + " (all scenes per launch)",
+                     "bytes_per_lu": kb, "avg_ms": round(avg_s * 1e3, 4),
+                     "timed_over": f"{ksteps} steps with HIP events after the timed call"},
         "whole_step_hbm_frac": round(STEP_BYTES_PER_LU[args.storage] * mlups * 1e6 /
                                      (HBM_PEAK_GBS * 1e9), 4),
         "copy_probe_gbps": round(copy_gbps, 1),
         "kernel_ms": {k: {"total": round(v[0], 3), "launches": v[1]} for k, v in kt.items()},
         "finite": finite,
+        "autotune": tune,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -260,6 +275,8 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=160.0, help="serve: GiB of scene state")
     ap.add_argument("--max-active", type=int, default=48, help="serve: active scene cap")
     ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--tune-steps", type=int, default=10,
+                    help="batch: autotuner trial steps per candidate (0: no autotuning)")
     ap.add_argument("--storage", choices=["f32", "f16", "bf16"], default="f32",
                     help="population storage: f32 (reference-exact, FP64 arithmetic) or a "
                          "16-bit deviation (FP32 arithmetic)")
@@ -322,7 +339,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": measured_traffic(n, args.storage),
-                     "kernel": "k1_stream_collide", "bytes_per_lu": kb,
+                     "kernel": STREAM_KERNEL[args.storage], "bytes_per_lu": kb,
                      "avg_ms": round(avg_s * 1e3, 4)},
         "whole_step_hbm_frac": round(STEP_BYTES_PER_LU[args.storage] * mlups * 1e6 /
                                      (HBM_PEAK_GBS * 1e9), 4),

@@ -59,29 +59,45 @@ struct Codec<ST_F32> {
   __device__ __forceinline__ static float rt(float v, float) { return v; }
 };
 
+// float -> binary16, round to nearest even, subnormals kept, as one
+// v_cvt_f16_f32 of a float held in a register.  Written out so that every
+// rounding to 16 bits is this instruction: the compiler would otherwise fold
+// some conversions into mixed-precision FMAs (v_fma_mixlo_f16), which round the
+// product-sum to 16 bits once instead of first to float, and whether it does
+// depends on the surrounding code -- the same arithmetic in two kernels (the
+// fused step, and a call split between stream and collide) would then store
+// different bits where the float result falls on a 16-bit rounding tie (mostly
+// subnormal deviations, in the gas).  The result's upper 16 bits are masked.
+__device__ __forceinline__ uint16_t f32_to_f16_bits(float v) {
+  uint32_t r;
+  asm("v_cvt_f16_f32 %0, %1" : "=v"(r) : "v"(v));
+  return (uint16_t)(r & 0xffffu);
+}
+
 template <>
 struct Codec<ST_F16> {
   using T = _Float16;
   using R = float;
   static constexpr int bytes = 2;
+  __device__ __forceinline__ static T h(float v) { return __builtin_bit_cast(T, f32_to_f16_bits(v)); }
   __device__ __forceinline__ static float ld(const T *p, int64_t a, float base) {
     return (float)p[a] + base;
   }
   __device__ __forceinline__ static void st(T *p, int64_t a, float v, float base) {
-    p[a] = (T)(v - base);
+    p[a] = h(v - base);
   }
   __device__ __forceinline__ static float ldo(const T *p, uint32_t bo, float base) {
     return (float)ldb(p, bo) + base;
   }
   __device__ __forceinline__ static void sto(T *p, uint32_t bo, float v, float base) {
-    stb(p, bo, (T)(v - base));
+    stb(p, bo, h(v - base));
   }
   __device__ __forceinline__ static float rt(float v, float base) {
-    return (float)(T)(v - base) + base;
+    return (float)h(v - base) + base;
   }
   // The stored bits of an element: dec(enc(v, b), b) == ld after st.
   __device__ __forceinline__ static uint16_t enc(float v, float base) {
-    return __builtin_bit_cast(uint16_t, (T)(v - base));
+    return f32_to_f16_bits(v - base);
   }
   __device__ __forceinline__ static float dec(uint16_t bits, float base) {
     return (float)__builtin_bit_cast(T, bits) + base;
@@ -92,10 +108,10 @@ struct Codec<ST_F16> {
   __device__ __forceinline__ static float dv(uint16_t bits) {
     return (float)__builtin_bit_cast(T, bits);
   }
-  __device__ __forceinline__ static uint16_t ev(float v) { return __builtin_bit_cast(uint16_t, (T)v); }
-  __device__ __forceinline__ static float rtv(float v) { return (float)(T)v; }
+  __device__ __forceinline__ static uint16_t ev(float v) { return f32_to_f16_bits(v); }
+  __device__ __forceinline__ static float rtv(float v) { return (float)h(v); }
   __device__ __forceinline__ static float ldd(const T *p, uint32_t bo) { return (float)ldb(p, bo); }
-  __device__ __forceinline__ static void std_(T *p, uint32_t bo, float v) { stb(p, bo, (T)v); }
+  __device__ __forceinline__ static void std_(T *p, uint32_t bo, float v) { stb(p, bo, h(v)); }
 };
 
 template <>

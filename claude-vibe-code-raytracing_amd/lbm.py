@@ -85,6 +85,15 @@ class ServerStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TuneResult(C.Structure):
+    """lbm_tune_result: the launch configuration the autotuner chose."""
+    _fields_ = [("cells_per_lane", C.c_int), ("graph_steps", C.c_int),
+                ("ms_per_step", C.c_double), ("candidates", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -181,6 +190,8 @@ def lib():
         "lbm_batch_set_graph_steps": (C.c_int, [P, C.c_int]),
         "lbm_batch_graph_counts": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "lbm_batch_set_stream_lanes": (C.c_int, [P, C.c_int]),
+        "lbm_batch_autotune": (C.c_int, [P, C.c_int, C.POINTER(TuneResult)]),
+        "lbm_autotune": (C.c_int, [P, SP, C.c_int, C.POINTER(TuneResult)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -424,6 +435,14 @@ class Sim:
         """Cells per lane (1 or 2) of the 16-bit stream kernel."""
         self._ck(self.L.lbm_set_stream_lanes(self.h, int(cells_per_lane)), "lbm_set_stream_lanes")
 
+    def autotune(self, sphere=None, trial_steps=10):
+        """Times the launch configurations over real steps (every one computes
+        the same state), keeps the fastest; returns it as a dict."""
+        r = TuneResult()
+        self._ck(self.L.lbm_autotune(self.h, C.byref(sphere) if sphere is not None else None,
+                                     int(trial_steps), C.byref(r)), "lbm_autotune")
+        return r.as_dict()
+
     def graph_counts(self):
         """(captures, replays) of launch graphs since the handle was created."""
         a, b = C.c_int64(0), C.c_int64(0)
@@ -489,6 +508,13 @@ class Batch:
     def set_stream_lanes(self, cells_per_lane):
         self._ck(self.L.lbm_batch_set_stream_lanes(self.h, int(cells_per_lane)),
                  "lbm_batch_set_stream_lanes")
+
+    def autotune(self, trial_steps=10):
+        """lbm_batch_autotune: returns the chosen configuration as a dict."""
+        r = TuneResult()
+        self._ck(self.L.lbm_batch_autotune(self.h, int(trial_steps), C.byref(r)),
+                 "lbm_batch_autotune")
+        return r.as_dict()
 
     def graph_counts(self):
         a, b = C.c_int64(0), C.c_int64(0)

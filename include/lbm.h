@@ -233,6 +233,25 @@ int lbm_batch_graph_counts(const lbm_batch *batch, int64_t *captures, int64_t *r
 /* Cells per lane of the batch's 16-bit stream kernel, as lbm_set_stream_lanes. */
 int lbm_batch_set_stream_lanes(lbm_batch *batch, int cells_per_lane);
 
+/* Launch-shape autotuner.  No reference counterpart.  Times every candidate
+ * launch configuration of the batch -- cells per lane of the 16-bit stream
+ * kernel (when a member stores 16-bit populations) and steps per captured
+ * graph (0, 2, 8, 32, those below trial_steps - 1) -- over trial_steps real
+ * coupled steps each, after an untimed lead-in of 2 steps, keeps the fastest
+ * and reports it.  Every candidate computes the same state, so the trials
+ * simply advance the members: after the call each has advanced by
+ * candidates * (trial_steps + 2) steps, exactly as lbm_batch_step would have
+ * moved it. */
+typedef struct {
+  int cells_per_lane;  /* 1 or 2 */
+  int graph_steps;     /* 0: kernels launched one by one */
+  double ms_per_step;  /* of the chosen configuration */
+  int candidates;      /* configurations timed */
+} lbm_tune_result;
+int lbm_batch_autotune(lbm_batch *batch, int trial_steps, lbm_tune_result *out);
+/* The same for one simulation's lbm_step. */
+int lbm_autotune(lbm_sim *sim, lbm_sphere *solid, int trial_steps, lbm_tune_result *out);
+
 /* ------------------------------------------------------------------ serving */
 /* A request loop that keeps many scenes resident and steps them with
  * continuous batching.  No reference counterpart as an API: it serves the

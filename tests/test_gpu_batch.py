@@ -86,3 +86,24 @@ def test_batch_split_call_equals_single_call():
     for n in (1, 7, 2, 20):
         sim2.coupled(b2, n)
     assert_same(snapshot(sim2, b2), snapshot(sim1, b1), "split")
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("calls", [(18, 19), (17, 20), (2, 6, 2, 6, 2, 6, 2, 6, 5)])
+def test_split_calls_16bit(storage, calls):
+    """With 16-bit storage a call split between the stream and the collide
+    stores and reloads the same 16-bit values the fused step keeps, so any
+    split gives the one-call state (config 0's sphere crosses cells near steps
+    17-19, and the gas holds subnormal deviations)."""
+    announce("16-bit split calls", storage, calls)
+    total = sum(calls)
+    sim1, b1 = make(SCENES[0])
+    sim1.set_storage(storage)
+    sim1.coupled(b1, total)
+    sim2, b2 = make(SCENES[0])
+    sim2.set_storage(storage)
+    for n in calls:
+        sim2.coupled(b2, n)
+    assert_same(snapshot(sim2, b2), snapshot(sim1, b1), f"{storage} split {calls}")
+    sim1.close()
+    sim2.close()

@@ -80,3 +80,35 @@ def test_graph_steps_argument_checked():
     sim.set_graph_steps(0)
     sim.set_graph_steps(64)
     sim.close()
+
+
+def test_autotune_steps_like_plain_calls():
+    """The autotuner's trials are ordinary steps: afterwards every member is
+    where plain calls of the same total length leave it (interior fluid
+    populations and every other field, bit for bit)."""
+    from test_gpu_batch import assert_same, snapshot
+    announce("autotune vs plain calls")
+    subset = [0, 1, 2]
+    tuned = lbm.Batch()
+    sims = [make(SCENES[j]) for j in subset]
+    for sim, b in sims:
+        sim.set_storage("f16")
+        tuned.add(sim, b)
+    r = tuned.autotune(6)
+    assert r["cells_per_lane"] in (1, 2) and r["graph_steps"] in (0, 2, 8, 32)
+    assert r["candidates"] == 4 and r["ms_per_step"] > 0, r   # 2 lane counts x graph steps 0, 2
+    total = r["candidates"] * (6 + 2)
+    tuned.step(5)
+    plain = lbm.Batch()
+    refs = [make(SCENES[j]) for j in subset]
+    for sim, b in refs:
+        sim.set_storage("f16")
+        plain.add(sim, b)
+    plain.step(total)
+    plain.step(5)
+    for j, ((s1, b1), (s2, b2)) in enumerate(zip(sims, refs)):
+        assert_same(snapshot(s1, b1), snapshot(s2, b2), f"scene {subset[j]}")
+    for bt in (tuned, plain):
+        bt.close()
+    for sim, _ in sims + refs:
+        sim.close()
