@@ -599,7 +599,7 @@ struct Scene {
   int live;                 // continues a call that ended after a step's body update
   int use_bf;               // external force armed for the first step of the call
   int cgx, cgy;             // full-grid blocks of 64x4 cells: cgx * cgy * nz
-  int bgx, bgy, bgz;        // band blocks of 64x4 cells (covers the largest band box)
+  int band_blocks;          // blocks of 256 cells covering the largest band box
 };
 
 // Buffer of parity q (0 or 1) of a pair, as codec elements.  Both pointers are
@@ -627,16 +627,23 @@ __device__ __forceinline__ int find_scene(const int *__restrict__ off, int n, in
   return lo;
 }
 
-// Cell of block lb of a 64x4-block launch over the whole grid or over the band
-// box; false when outside.
+// Cell of block lb of a launch over the whole grid (blocks of 64x4 cells) or
+// over the band box (blocks of 256 box cells); false when outside.
 template <bool BAND>
 __device__ __forceinline__ bool block_cell(const Scene &sc, int lb, int &x, int &y, int &z) {
   if (BAND) {
+    // The box's cells numbered x fastest, 256 per block, so narrow boxes
+    // leave no lanes idle.
     const int *box = sc.ctrl->box_band;
-    x = box[0] + (lb % sc.bgx) * 64 + threadIdx.x;
-    y = box[2] + ((lb / sc.bgx) % sc.bgy) * 4 + threadIdx.y;
-    z = box[4] + lb / (sc.bgx * sc.bgy);
-    return x <= box[1] && y <= box[3] && z <= box[5];
+    const int ex = box[1] - box[0] + 1, ey = box[3] - box[2] + 1, ez = box[5] - box[4] + 1;
+    if (ex <= 0 || ey <= 0 || ez <= 0) return false;
+    const int t = lb * 256 + (int)threadIdx.y * 64 + (int)threadIdx.x;
+    if (t >= ex * ey * ez) return false;
+    int r;
+    z = box[4] + small_divmod(t, ex * ey, r);
+    y = box[2] + small_divmod(r, ex, x);
+    x += box[0];
+    return true;
   }
   x = (lb % sc.cgx) * 64 + threadIdx.x;
   y = ((lb / sc.cgx) % sc.cgy) * 4 + threadIdx.y;
@@ -831,22 +838,18 @@ __device__ __forceinline__ void phase_stream_body(const Scene &sc, int lb, int q
   if constexpr (KIND != ST_F32) {
     // The stream kernel's phi(t+1) of the cell (phase_sum_dev), from the same
     // pulled deviations and baselines.
+    // Each address chosen without branching (own opposite population at a
+    // non-fluid source): one load per direction, all in flight together, also
+    // in waves that hold wall cells (most waves of a narrow grid).
     using C = Codec<KIND>;
     const Phi7 v = load_phi7(g, phi, bo4);
-    const uint32_t bo = (uint32_t)i * 2u;
     float e[Q7], phib[Q7], hfull[Q7];
 #pragma unroll
     for (int k = 0; k < Q7; k++) {
-      e[k] = C::ldd(h2 + ((int64_t)k * g.S - dir_off(g, k)), bo);
-      phib[k] = v.src(k);
-    }
-    if (nonfluid) {
-#pragma unroll
-      for (int k = 1; k < Q7; k++)
-        if ((nonfluid >> k) & 1u) {
-          e[k] = C::ldd(h2 + (int64_t)opp7(k) * g.S, bo);
-          phib[k] = v.c;
-        }
+      const bool bounce = (nonfluid >> k) & 1u;
+      const int64_t a = bounce ? opp7(k) * g.S + i : k * g.S + i - dir_off(g, k);
+      e[k] = C::dv(__builtin_bit_cast(uint16_t, h2[a]));
+      phib[k] = bounce ? v.c : v.src(k);
     }
     stb(phi_next, bo4, phase_sum_dev(e, phib, hfull));
   } else {
@@ -859,12 +862,12 @@ __device__ __forceinline__ void phase_stream_body(const Scene &sc, int lb, int q
     stb(phi_next, bo4, (float)pp);
   }
 }
+// Tile launches over a scene set: tiles[b] = (scene, tile of the scene) of block b.
 template <int KIND>
 __global__ __launch_bounds__(256) void km_phase_stream(const Scene *__restrict__ scenes,
-                                                       const int *__restrict__ off, int nscenes,
-                                                       int q) {
-  const int si = find_scene(off, nscenes, blockIdx.x);
-  phase_stream_body<KIND>(scenes[si], blockIdx.x - off[si], q);
+                                                       const int2 *__restrict__ tiles, int q) {
+  const int2 t = tiles[blockIdx.x];
+  phase_stream_body<KIND>(scenes[t.x], t.y, q);
 }
 template <int KIND>
 __global__ __launch_bounds__(256) void k1_phase_stream(const Scene sc, int q) {
@@ -1158,10 +1161,10 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
 }
 template <bool COLLIDE, int KIND>
 __global__ __launch_bounds__(256) void km_stream_collide(const Scene *__restrict__ scenes,
-                                                         const int *__restrict__ off, int nscenes,
-                                                         int q, int first) {
-  const int si = find_scene(off, nscenes, blockIdx.x);
-  stream_collide_body<COLLIDE, KIND>(scenes[si], blockIdx.x - off[si], q, first);
+                                                         const int2 *__restrict__ tiles, int q,
+                                                         int first) {
+  const int2 t = tiles[blockIdx.x];
+  stream_collide_body<COLLIDE, KIND>(scenes[t.x], t.y, q, first);
 }
 template <bool COLLIDE, int KIND>
 __global__ __launch_bounds__(256) void k1_stream_collide(const Scene sc, int q, int first) {
@@ -1438,11 +1441,10 @@ __device__ __forceinline__ void stream_collide_pair(const Scene &sc, int lb, int
   else stream_collide_pair_body<COLLIDE, KIND, 2>(sc, tile, q, first, red);
 }
 template <bool COLLIDE, int KIND>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void km_stream_collide2(const Scene *__restrict__ scenes,
-                                                          const int *__restrict__ off, int nscenes,
-                                                          int q, int first) {
-  const int si = find_scene(off, nscenes, blockIdx.x);
-  stream_collide_pair<COLLIDE, KIND>(scenes[si], blockIdx.x - off[si], q, first);
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void km_stream_collide2(
+    const Scene *__restrict__ scenes, const int2 *__restrict__ tiles, int q, int first) {
+  const int2 t = tiles[blockIdx.x];
+  stream_collide_pair<COLLIDE, KIND>(scenes[t.x], t.y, q, first);
 }
 template <bool COLLIDE, int KIND>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k1_stream_collide2(const Scene sc, int q, int first) {
