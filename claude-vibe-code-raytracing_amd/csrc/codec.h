@@ -40,6 +40,15 @@ template <typename T>
 __device__ __forceinline__ void stb(T *base, uint32_t bo, T v) {
   *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + bo) = v;
 }
+// A population store marked non-temporal: no step reads the value back before
+// the whole grid has streamed through the caches, so the line should not take
+// L2 room from the phi stencils and the pulls' halo lines (measured on MI355X,
+// 256^3: -6% on the float32 stream kernel, -3% on the FP16 one, and a faster
+// phase stream after them).
+template <typename T>
+__device__ __forceinline__ void stn(T *base, uint32_t bo, T v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<T *>(reinterpret_cast<char *>(base) + bo));
+}
 
 template <int KIND>
 struct Codec;
@@ -54,7 +63,7 @@ struct Codec<ST_F32> {
   __device__ __forceinline__ static void st(T *p, int64_t a, float v, float) { p[a] = v; }
   // ld / st at a byte offset from a uniform base (ldb / stb).
   __device__ __forceinline__ static float ldo(const T *p, uint32_t bo, float) { return ldb(p, bo); }
-  __device__ __forceinline__ static void sto(T *p, uint32_t bo, float v, float) { stb(p, bo, v); }
+  __device__ __forceinline__ static void sto(T *p, uint32_t bo, float v, float) { stn(p, bo, v); }
   // The value st then ld give back.
   __device__ __forceinline__ static float rt(float v, float) { return v; }
 };
@@ -90,7 +99,7 @@ struct Codec<ST_F16> {
     return (float)ldb(p, bo) + base;
   }
   __device__ __forceinline__ static void sto(T *p, uint32_t bo, float v, float base) {
-    stb(p, bo, h(v - base));
+    stn(reinterpret_cast<uint16_t *>(p), bo, f32_to_f16_bits(v - base));
   }
   __device__ __forceinline__ static float rt(float v, float base) {
     return (float)h(v - base) + base;
@@ -111,7 +120,9 @@ struct Codec<ST_F16> {
   __device__ __forceinline__ static uint16_t ev(float v) { return f32_to_f16_bits(v); }
   __device__ __forceinline__ static float rtv(float v) { return (float)h(v); }
   __device__ __forceinline__ static float ldd(const T *p, uint32_t bo) { return (float)ldb(p, bo); }
-  __device__ __forceinline__ static void std_(T *p, uint32_t bo, float v) { stb(p, bo, h(v)); }
+  __device__ __forceinline__ static void std_(T *p, uint32_t bo, float v) {
+    stn(reinterpret_cast<uint16_t *>(p), bo, f32_to_f16_bits(v));
+  }
 };
 
 template <>
@@ -142,7 +153,7 @@ struct Codec<ST_BF16> {
   __device__ __forceinline__ static uint16_t ev(float v) { return f32_to_bf16(v); }
   __device__ __forceinline__ static float rtv(float v) { return bf16_to_f32(f32_to_bf16(v)); }
   __device__ __forceinline__ static float ldd(const T *p, uint32_t bo) { return bf16_to_f32(ldb(p, bo)); }
-  __device__ __forceinline__ static void std_(T *p, uint32_t bo, float v) { stb(p, bo, f32_to_bf16(v)); }
+  __device__ __forceinline__ static void std_(T *p, uint32_t bo, float v) { stn(p, bo, f32_to_bf16(v)); }
 };
 
 }  // namespace lbm

@@ -1225,11 +1225,17 @@ constexpr int kRsrcWord3 = 0x00020000;
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t pop_rsrc(const void *p, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, kRsrcWord3);
 }
+// Cache policy of the pair loads and stores: stores are non-temporal (see stn);
+// loads too in the one-scene form (NTL), where a grid's rows are long enough
+// that the pulls' halo lines are not worth keeping (measured: 0.63 -> 0.60 ms at
+// 256^3), but not for sets of small scenes, whose narrow rows share them.
+constexpr int kCacheNT = 2;  // the nt bit of the cache-policy operand on gfx950
+template <bool NTL = false>
 __device__ __forceinline__ uint32_t ld_pair(__amdgpu_buffer_rsrc_t r, uint32_t e) {
-  return __builtin_amdgcn_raw_buffer_load_b32(r, (int)(2u * e), 0, 0);
+  return __builtin_amdgcn_raw_buffer_load_b32(r, (int)(2u * e), 0, NTL ? kCacheNT : 0);
 }
 __device__ __forceinline__ void st_pair(__amdgpu_buffer_rsrc_t r, uint32_t e, uint32_t v) {
-  __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)(2u * e), 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)(2u * e), 0, kCacheNT);
 }
 
 // Elements p0 + i0 - off and p0 + i0 + 1 - off of a buffer of 16-bit elements
@@ -1238,14 +1244,14 @@ __device__ __forceinline__ void st_pair(__amdgpu_buffer_rsrc_t r, uint32_t e, ui
 // offset past its end).  last: wbase + 127, the second cell of the wave's last
 // lane; for odd off every lane loads that lane's upper pair from the same
 // address (one request).
-template <bool ODD>
+template <bool ODD, bool NTL>
 __device__ __forceinline__ uint32_t pull_pair(__amdgpu_buffer_rsrc_t r, uint32_t p0, uint32_t i0,
                                               uint32_t off, uint32_t last, int lane) {
   if constexpr (!ODD) {
-    return ld_pair(r, p0 + i0 - off);
+    return ld_pair<NTL>(r, p0 + i0 - off);
   } else {
-    const uint32_t w = ld_pair(r, p0 + i0 - off - 1u);
-    const uint32_t edge = ld_pair(r, p0 + last - off);
+    const uint32_t w = ld_pair<NTL>(r, p0 + i0 - off - 1u);
+    const uint32_t edge = ld_pair<NTL>(r, p0 + last - off);
     uint32_t nxt = (uint32_t)__shfl_down((int)w, 1, 64);
     nxt = lane == 63 ? edge : nxt;
     return (w >> 16) | (nxt << 16);
@@ -1253,7 +1259,7 @@ __device__ __forceinline__ uint32_t pull_pair(__amdgpu_buffer_rsrc_t r, uint32_t
 }
 
 // red: 6 x 256 doubles of workgroup memory for the tile's link momentum.
-template <bool COLLIDE, int KIND, int PC>
+template <bool COLLIDE, int KIND, int PC, bool NTL>
 __device__ __forceinline__ void stream_collide_pair_body(const Scene &sc, int tile, int q,
                                                          int first, double (*red)[256]) {
   using C = Codec<KIND>;
@@ -1308,11 +1314,11 @@ __device__ __forceinline__ void stream_collide_pair_body(const Scene &sc, int ti
   uint32_t fr[Q27], hr[Q7];
   static_for<0, Q27>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    fr[k] = pull_pair<PullOdd<PC, k>::v>(f2, k * S32, i0u, (uint32_t)dir_off(g, k), last, lane);
+    fr[k] = pull_pair<PullOdd<PC, k>::v, NTL>(f2, k * S32, i0u, (uint32_t)dir_off(g, k), last, lane);
   });
   static_for<0, Q7>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    hr[k] = pull_pair<PullOdd<PC, k>::v>(h2, k * S32, i0u, (uint32_t)dir_off(g, k), last, lane);
+    hr[k] = pull_pair<PullOdd<PC, k>::v, NTL>(h2, k * S32, i0u, (uint32_t)dir_off(g, k), last, lane);
   });
   // Halfway bounce-back: the own opposite population where a source is not
   // fluid, requested for every such direction before any is merged.
@@ -1321,10 +1327,10 @@ __device__ __forceinline__ void stream_collide_pair_body(const Scene &sc, int ti
     uint32_t ob[Q27], oh[Q7];
 #pragma unroll
     for (int k = 1; k < Q27; k++)
-      ob[k] = ((nfa >> k) & 1u) ? ld_pair(f2, opp27(k) * S32 + i0u) : 0u;
+      ob[k] = ((nfa >> k) & 1u) ? ld_pair<NTL>(f2, opp27(k) * S32 + i0u) : 0u;
 #pragma unroll
     for (int k = 1; k < Q7; k++)
-      oh[k] = ((nfa >> k) & 1u) ? ld_pair(h2, opp7(k) * S32 + i0u) : 0u;
+      oh[k] = ((nfa >> k) & 1u) ? ld_pair<NTL>(h2, opp7(k) * S32 + i0u) : 0u;
 #pragma unroll
     for (int k = 1; k < Q27; k++) {
       const uint32_t bm = (((nf[0] >> k) & 1u) ? 0x0000ffffu : 0u) |
@@ -1430,25 +1436,25 @@ __device__ __forceinline__ void stream_collide_pair_body(const Scene &sc, int ti
 // that four waves share a SIMD (measured on MI355X: 0.68 against 0.72 ms at 3
 // waves for the 256^3 scene in FP16).
 constexpr int kPairWaves = 2;
-template <bool COLLIDE, int KIND>
+template <bool COLLIDE, int KIND, bool NTL>
 __device__ __forceinline__ void stream_collide_pair(const Scene &sc, int lb, int q, int first) {
   int tile;
   if (!map_tile(sc.tm.T, sc.tm.T8, lb, tile)) return;
   __shared__ double red[6][256];
   const int pc = parity_class(sc.g.nx, sc.g.ny);
-  if (pc == 0) stream_collide_pair_body<COLLIDE, KIND, 0>(sc, tile, q, first, red);
-  else if (pc == 1) stream_collide_pair_body<COLLIDE, KIND, 1>(sc, tile, q, first, red);
-  else stream_collide_pair_body<COLLIDE, KIND, 2>(sc, tile, q, first, red);
+  if (pc == 0) stream_collide_pair_body<COLLIDE, KIND, 0, NTL>(sc, tile, q, first, red);
+  else if (pc == 1) stream_collide_pair_body<COLLIDE, KIND, 1, NTL>(sc, tile, q, first, red);
+  else stream_collide_pair_body<COLLIDE, KIND, 2, NTL>(sc, tile, q, first, red);
 }
 template <bool COLLIDE, int KIND>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void km_stream_collide2(
     const Scene *__restrict__ scenes, const int2 *__restrict__ tiles, int q, int first) {
   const int2 t = tiles[blockIdx.x];
-  stream_collide_pair<COLLIDE, KIND>(scenes[t.x], t.y, q, first);
+  stream_collide_pair<COLLIDE, KIND, false>(scenes[t.x], t.y, q, first);
 }
 template <bool COLLIDE, int KIND>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k1_stream_collide2(const Scene sc, int q, int first) {
-  stream_collide_pair<COLLIDE, KIND>(sc, blockIdx.x, q, first);
+  stream_collide_pair<COLLIDE, KIND, true>(sc, blockIdx.x, q, first);
 }
 
 // ---------------------------------------------------------------- storage conversion
