@@ -1182,8 +1182,8 @@ __global__ __launch_bounds__(256) void k1_stream_collide(const Scene sc, int q, 
 //
 // The source pair of a direction, (i0 - off, i0 + 1 - off), is an aligned pair
 // when off is even.  When off is odd it is the upper element of the aligned pair
-// at i0 - off - 1 and the lower element of the next lane's; the wave's last lane
-// loads that pair itself.  The parity of off follows from the parities of nx and
+// at i0 - off - 1 and the lower element of the aligned pair after it; the lane
+// loads both.  The parity of off follows from the parities of nx and
 // nx ny (the parity class of the scene), so it is known per direction at
 // compile time and all lanes take the same path.
 //
@@ -1241,21 +1241,27 @@ __device__ __forceinline__ void st_pair(__amdgpu_buffer_rsrc_t r, uint32_t e, ui
 // Elements p0 + i0 - off and p0 + i0 + 1 - off of a buffer of 16-bit elements
 // (p0: the first element of the plane), packed, the first in the low half.
 // Element indices wrap modulo 2^32 (a source before the buffer becomes an
-// offset past its end).  last: wbase + 127, the second cell of the wave's last
-// lane; for odd off every lane loads that lane's upper pair from the same
-// address (one request).
+// offset past its end).  For odd off the two elements straddle the aligned
+// pairs (src - 1, src) and (src + 1, src + 2), and the lane loads both (lo,
+// hi) and joins them with pull_join once every pull is in flight: no lane
+// exchange, so the pulls of a cell are independent loads.  For a direction
+// k >= 1 the lower pair's element index, k S + i0 - off - 1, is at least S - 2,
+// so it does not wrap.
 template <bool ODD, bool NTL>
-__device__ __forceinline__ uint32_t pull_pair(__amdgpu_buffer_rsrc_t r, uint32_t p0, uint32_t i0,
-                                              uint32_t off, uint32_t last, int lane) {
+__device__ __forceinline__ void pull_pair(__amdgpu_buffer_rsrc_t r, uint32_t p0, uint32_t i0,
+                                          uint32_t off, uint32_t &lo, uint32_t &hi) {
   if constexpr (!ODD) {
-    return ld_pair<NTL>(r, p0 + i0 - off);
+    lo = ld_pair<NTL>(r, p0 + i0 - off);
   } else {
-    const uint32_t w = ld_pair<NTL>(r, p0 + i0 - off - 1u);
-    const uint32_t edge = ld_pair<NTL>(r, p0 + last - off);
-    uint32_t nxt = (uint32_t)__shfl_down((int)w, 1, 64);
-    nxt = lane == 63 ? edge : nxt;
-    return (w >> 16) | (nxt << 16);
+    const uint32_t e = p0 + i0 - off - 1u;
+    lo = ld_pair<NTL>(r, e);
+    hi = ld_pair<NTL>(r, e + 2u);
   }
+}
+template <bool ODD>
+__device__ __forceinline__ uint32_t pull_join(uint32_t lo, uint32_t hi) {
+  if constexpr (!ODD) return lo;
+  else return (lo >> 16) | (hi << 16);
 }
 
 // red: 6 x 256 doubles of workgroup memory for the tile's link momentum.
@@ -1272,7 +1278,7 @@ __device__ __forceinline__ void stream_collide_pair_body(const Scene &sc, int ti
   const int lane = threadIdx.x;
   const int64_t wbase = ((int64_t)tile << kTileShift) + 128 * (int)threadIdx.y;
   const int64_t i0 = wbase + 2 * lane;
-  const uint32_t i0u = (uint32_t)i0, last = (uint32_t)(wbase + 127);
+  const uint32_t i0u = (uint32_t)i0;
   q &= 1;
   const __amdgpu_buffer_rsrc_t f2 = pop_rsrc(par(sc.f, q), (uint32_t)(Q27 * 2 * S));
   const __amdgpu_buffer_rsrc_t h2 = pop_rsrc(par(sc.h, q), (uint32_t)(Q7 * 2 * S));
@@ -1310,15 +1316,26 @@ __device__ __forceinline__ void stream_collide_pair_body(const Scene &sc, int ti
       else nf[J] = wall_mask(g, xs[J], ys[J], zs[J]);
     }
   }
-  // Pulled pairs.
-  uint32_t fr[Q27], hr[Q7];
+  // Pulled pairs: every load is issued before any is joined, so all of them
+  // are in flight together (the loads take 54 registers at most).  Joined
+  // inside each pull, the compiler waited for every odd pull's loads before
+  // issuing the next direction's.
+  uint32_t fr[Q27], hr[Q7], fx[Q27], hx[Q7];
   static_for<0, Q27>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    fr[k] = pull_pair<PullOdd<PC, k>::v, NTL>(f2, k * S32, i0u, (uint32_t)dir_off(g, k), last, lane);
+    pull_pair<PullOdd<PC, k>::v, NTL>(f2, k * S32, i0u, (uint32_t)dir_off(g, k), fr[k], fx[k]);
   });
   static_for<0, Q7>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    hr[k] = pull_pair<PullOdd<PC, k>::v, NTL>(h2, k * S32, i0u, (uint32_t)dir_off(g, k), last, lane);
+    pull_pair<PullOdd<PC, k>::v, NTL>(h2, k * S32, i0u, (uint32_t)dir_off(g, k), hr[k], hx[k]);
+  });
+  static_for<0, Q27>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    fr[k] = pull_join<PullOdd<PC, k>::v>(fr[k], fx[k]);
+  });
+  static_for<0, Q7>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    hr[k] = pull_join<PullOdd<PC, k>::v>(hr[k], hx[k]);
   });
   // Halfway bounce-back: the own opposite population where a source is not
   // fluid, requested for every such direction before any is merged.
