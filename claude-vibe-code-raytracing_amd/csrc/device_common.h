@@ -70,6 +70,9 @@ struct TileMap {
 };
 
 constexpr int kTileShift = 8;  // 256 cells per tile: one cell per lane
+// Largest plane stride the fused float32 kernels address (two planes of 4-byte
+// elements within a 32-bit byte offset).
+constexpr int64_t kF32FusedMaxCells = (int64_t)1 << 29;
 
 inline TileMap make_tile_map(int nx, int ny, int nz) {
   TileMap m;

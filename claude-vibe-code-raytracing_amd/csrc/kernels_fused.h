@@ -433,6 +433,27 @@ __device__ __forceinline__ void links27(const Geom &g, int64_t i, const float ra
   }
 }
 
+// Element pulled for direction k of a cell at byte offset bo = i * sizeof(T):
+// the source plane's element i - off, or at a non-fluid source the cell's own
+// element of the opposite direction.  A direction and its opposite hold
+// adjacent planes (k odd: k, k + 1), so both candidates lie within two planes
+// of the uniform base at the lower of them, and the choice is a 32-bit byte
+// offset made per lane without branching: the address needs no 64-bit
+// arithmetic and all pulls of a cell are in flight together.  Needs
+// 2 S sizeof(T) < 2^32 (float32 scenes: S < 2^29), and for the source i >= off,
+// which holds for fluid cells (their sources lie in the grid).
+template <typename T>
+__device__ __forceinline__ T pull_elem(const T *__restrict__ p2, int64_t S, int k, int off,
+                                       uint32_t bo, bool bounce) {
+  if (k == 0) return ldb(p2, bo);
+  const bool lower = (k & 1) != 0;  // k is the lower plane of its pair
+  const T *base = p2 + (int64_t)(lower ? k : k - 1) * S;
+  const uint32_t plane = (uint32_t)S * (uint32_t)sizeof(T);
+  const uint32_t src = bo - (uint32_t)off * (uint32_t)sizeof(T) + (lower ? 0u : plane);
+  const uint32_t own = bo + (lower ? plane : 0u);
+  return ldb(base, bounce ? own : src);
+}
+
 // Pull stream of the 27 populations with halfway bounce-back, given the source
 // masks.  Every load address is chosen without branching (own opposite
 // population at a non-fluid source), so the 27 loads are in flight together.
@@ -449,14 +470,13 @@ __device__ __forceinline__ void pull27(const Geom &g, int64_t i,
                                        typename Codec<KIND>::R &pSumF,
                                        typename Codec<KIND>::R &mx, typename Codec<KIND>::R &my,
                                        typename Codec<KIND>::R &mz) {
+  static_assert(KIND == ST_F32, "16-bit codecs pull in deviation form");
   using C = Codec<KIND>;
   float raw[Q27];
+  const uint32_t bo = (uint32_t)i * 4u;
 #pragma unroll
-  for (int k = 0; k < Q27; k++) {
-    const int64_t s = i - ((int64_t)EZ27[k] * g.nxny + (int64_t)EY27[k] * g.nx + EX27[k]);
-    const int64_t a = ((nonfluid >> k) & 1u) ? opp27(k) * g.S + i : k * g.S + s;
-    raw[k] = C::ld(f2, a, w27(k));  // w27(opp(k)) == w27(k)
-  }
+  for (int k = 0; k < Q27; k++)
+    raw[k] = pull_elem(f2, g.S, k, (int)dir_off(g, k), bo, (nonfluid >> k) & 1u);
   links27<typename C::R>(g, i, raw, solid, usx, usy, usz, rhoPhaseOld, exchange, x, y, z, cx, cy,
                          cz, mom, fp, pSumF, mx, my, mz);
 }
@@ -522,23 +542,19 @@ __device__ __forceinline__ void links27_dev(const Geom &g, int64_t i, Get &&get,
   }
 }
 
-// Pull of the 7 phase populations (src/sim.js:376-397): from the source cell,
-// or the own opposite population when the source is not fluid (D3Q7 directions
-// 1..6 are D3Q27 directions 1..6, so the D3Q27 source mask serves).  A stored h
-// carries the phi of the cell it is stored at as its baseline: v holds phi of
-// the cell and its face neighbours (read only for 16-bit codecs).
+// Pull of the 7 float32 phase populations (src/sim.js:376-397): from the
+// source cell, or the own opposite population when the source is not fluid
+// (D3Q7 directions 1..6 are D3Q27 directions 1..6, so the D3Q27 source mask
+// serves).
 template <int KIND>
 __device__ __forceinline__ void pull7(const Geom &g, int64_t i,
                                       const typename Codec<KIND>::T *__restrict__ h2,
-                                      const Phi7 &v, uint32_t nonfluid, float hp[Q7]) {
-  using C = Codec<KIND>;
+                                      uint32_t nonfluid, float hp[Q7]) {
+  static_assert(KIND == ST_F32, "16-bit codecs pull in deviation form");
+  const uint32_t bo = (uint32_t)i * 4u;
 #pragma unroll
-  for (int k = 0; k < Q7; k++) {
-    const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
-    const bool bounce = (nonfluid >> k) & 1u;
-    const int64_t a = bounce ? opp7(k) * g.S + i : k * g.S + s;
-    hp[k] = C::ld(h2, a, KIND != ST_F32 ? w7(k) * (bounce ? v.c : v.src(k)) : 0.0f);
-  }
+  for (int k = 0; k < Q7; k++)
+    hp[k] = pull_elem(h2, g.S, k, (int)dir_off(g, k), bo, (nonfluid >> k) & 1u);
 }
 
 // Sum of a double over the 64 lanes of a wave by a fixed butterfly; every lane
@@ -844,17 +860,17 @@ __device__ __forceinline__ void phase_stream_body(const Scene &sc, int lb, int q
     using C = Codec<KIND>;
     const Phi7 v = load_phi7(g, phi, bo4);
     float e[Q7], phib[Q7], hfull[Q7];
+    const uint16_t *h16 = reinterpret_cast<const uint16_t *>(h2);
 #pragma unroll
     for (int k = 0; k < Q7; k++) {
       const bool bounce = (nonfluid >> k) & 1u;
-      const int64_t a = bounce ? opp7(k) * g.S + i : k * g.S + i - dir_off(g, k);
-      e[k] = C::dv(__builtin_bit_cast(uint16_t, h2[a]));
+      e[k] = C::dv(pull_elem(h16, g.S, k, (int)dir_off(g, k), (uint32_t)i * 2u, bounce));
       phib[k] = bounce ? v.c : v.src(k);
     }
     stb(phi_next, bo4, phase_sum_dev(e, phib, hfull));
   } else {
     float hv[Q7];
-    pull7<KIND>(g, i, h2, Phi7{}, nonfluid, hv);
+    pull7<KIND>(g, i, h2, nonfluid, hv);
     R phiSum = 0;
 #pragma unroll
     for (int k = 0; k < Q7; k++) phiSum += (R)hv[k];
@@ -1149,7 +1165,7 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
       pull27<KIND>(g, i, f2, nonfluid, solid, sc.usx, sc.usy, sc.usz, rhoPhaseOld, exchange, x, y,
                    z, cx, cy, cz, mom, fp, pSumF, mx, my, mz);
       float hp[Q7];
-      pull7<KIND>(g, i, h2, v, nonfluid, hp);
+      pull7<KIND>(g, i, h2, nonfluid, hp);
       const uint32_t bo = (uint32_t)i * (uint32_t)sizeof(T);
       finish_cell<COLLIDE, KIND>(
           sc, i, near, phi_next, gxo, gyo, gzo, lapo, fp, hp, pSumF, mx, my, mz, use_bf, phi_next,
@@ -1159,15 +1175,16 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
   }
   if (exchange) tile_momentum(mom, sc.partials, tile);
 }
+// Held to 128 VGPRs: four waves per SIMD.
 template <bool COLLIDE, int KIND>
-__global__ __launch_bounds__(256) void km_stream_collide(const Scene *__restrict__ scenes,
-                                                         const int2 *__restrict__ tiles, int q,
-                                                         int first) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void km_stream_collide(
+    const Scene *__restrict__ scenes, const int2 *__restrict__ tiles, int q, int first) {
   const int2 t = tiles[blockIdx.x];
   stream_collide_body<COLLIDE, KIND>(scenes[t.x], t.y, q, first);
 }
 template <bool COLLIDE, int KIND>
-__global__ __launch_bounds__(256) void k1_stream_collide(const Scene sc, int q, int first) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k1_stream_collide(
+    const Scene sc, int q, int first) {
   stream_collide_body<COLLIDE, KIND>(sc, blockIdx.x, q, first);
 }
 // ---------------------------------------------------------------- stream + collide, two cells per lane
