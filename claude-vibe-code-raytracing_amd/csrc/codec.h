@@ -123,6 +123,19 @@ struct Codec<ST_F16> {
   __device__ __forceinline__ static void std_(T *p, uint32_t bo, float v) {
     stn(reinterpret_cast<uint16_t *>(p), bo, f32_to_f16_bits(v));
   }
+  // ev(v) written into the low (HI false) or high half of a packed pair, the
+  // other half kept: the same v_cvt_f16_f32, with its result routed to the
+  // half by the instruction's destination select (one instruction instead of
+  // a conversion, a mask and a merge).
+  template <bool HI>
+  __device__ __forceinline__ static void put(uint32_t &w, float v) {
+    if constexpr (HI)
+      asm("v_cvt_f16_f32_sdwa %0, %1 dst_sel:WORD_1 dst_unused:UNUSED_PRESERVE src0_sel:DWORD"
+          : "+v"(w) : "v"(v));
+    else
+      asm("v_cvt_f16_f32_sdwa %0, %1 dst_sel:WORD_0 dst_unused:UNUSED_PRESERVE src0_sel:DWORD"
+          : "+v"(w) : "v"(v));
+  }
 };
 
 template <>
@@ -154,6 +167,12 @@ struct Codec<ST_BF16> {
   __device__ __forceinline__ static float rtv(float v) { return bf16_to_f32(f32_to_bf16(v)); }
   __device__ __forceinline__ static float ldd(const T *p, uint32_t bo) { return bf16_to_f32(ldb(p, bo)); }
   __device__ __forceinline__ static void std_(T *p, uint32_t bo, float v) { stn(p, bo, f32_to_bf16(v)); }
+  // ev(v) written into the low (HI false) or high half of a packed pair.
+  template <bool HI>
+  __device__ __forceinline__ static void put(uint32_t &w, float v) {
+    const uint32_t b = f32_to_bf16(v);
+    w = HI ? (w & 0x0000ffffu) | (b << 16) : (w & 0xffff0000u) | b;
+  }
 };
 
 }  // namespace lbm

@@ -1281,6 +1281,14 @@ __device__ __forceinline__ uint32_t pull_join(uint32_t lo, uint32_t hi) {
   else return (lo >> 16) | (hi << 16);
 }
 
+// A value stored into cell j (0: low half, 1: high half) of a packed pair; j is
+// a constant once the loop over the pair's cells is unrolled.
+template <typename C>
+__device__ __forceinline__ void put_half(uint32_t &w, int j, float v) {
+  if (j) C::template put<true>(w, v);
+  else C::template put<false>(w, v);
+}
+
 // red: 6 x 256 doubles of workgroup memory for the tile's link momentum.
 template <bool COLLIDE, int KIND, int PC, bool NTL>
 __device__ __forceinline__ void stream_collide_pair_body(const Scene &sc, int tile, int q,
@@ -1386,7 +1394,6 @@ __device__ __forceinline__ void stream_collide_pair_body(const Scene &sc, int ti
       double mom[6] = {0, 0, 0, 0, 0, 0};
     if (fl[J]) {
       const int sh = 16 * J;
-      const uint32_t keep = ~(0xffffu << sh);
       const uint32_t nonfluid = nf[J], solid = so[J];
       const int x = xs[J], y = ys[J], z = zs[J];
       const Phi7 v = load_phi7(g, phi, (uint32_t)i * 4u);
@@ -1400,7 +1407,7 @@ __device__ __forceinline__ void stream_collide_pair_body(const Scene &sc, int ti
       float sumD, mx, my, mz;
       links27_dev<KIND>(
           g, i, [&](int k) { return C::dv((uint16_t)(fr[k] >> sh)); },
-          [&](int k, float dk) { fr[k] = (fr[k] & keep) | ((uint32_t)C::ev(dk) << sh); }, solid,
+          [&](int k, float dk) { put_half<C>(fr[k], J, dk); }, solid,
           sc.usx, sc.usy, sc.usz, rhoPhaseOld, exchange, x, y, z, cx, cy, cz, mom, sumD, mx, my,
           mz);
       // A pulled h has the phi of its source as its baseline, a bounced one the
@@ -1415,8 +1422,8 @@ __device__ __forceinline__ void stream_collide_pair_body(const Scene &sc, int ti
           sc, i, nr[J], phi_next, gxo, gyo, gzo, lapo,
           [&](int k) { return C::dv((uint16_t)(fr[k] >> sh)); }, e, phib, sumD, mx, my, mz, use_bf,
           phi_next,
-          [&](int k, float dv) { fr[k] = (fr[k] & keep) | ((uint32_t)C::ev(dv) << sh); },
-          [&](int k, float dv) { hr[k] = (hr[k] & keep) | ((uint32_t)C::ev(dv) << sh); });
+          [&](int k, float dv) { put_half<C>(fr[k], J, dv); },
+          [&](int k, float dv) { put_half<C>(hr[k], J, dv); });
     } else if (!COLLIDE && i < g.n) {
       phi_next[i] = phi[i];
     }
