@@ -1498,6 +1498,124 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
   stream_collide_pair<COLLIDE, KIND, true>(sc, blockIdx.x, q, first);
 }
 
+// phase_stream_body for two cells per lane (16-bit codecs), with the pair
+// kernel's tiles, lanes and pulls: each direction's h pair is one aligned
+// 4-byte load (two when the pull offset is odd) and each source phi pair two
+// 4-byte loads, so a pair of cells takes about two thirds of the requests of
+// two cells alone.  The sum is phase_sum_dev of the same deviations and
+// baselines, so phi(t+1) is the one-cell kernel's bit for bit.  Loads go
+// through buffer resources: an offset outside a buffer (a wall cell's source
+// before the grid) reads 0 and is never used.
+template <int KIND, int PC>
+__device__ __forceinline__ void phase_stream_pair_body(const Scene &sc, int tile, int q) {
+  using C = Codec<KIND>;
+  static_assert(KIND != ST_F32, "16-bit codecs only");
+  const Geom &g = sc.g;
+  const int64_t S = g.S;
+  const uint32_t S32 = (uint32_t)S;
+  const int lane = threadIdx.x;
+  const int64_t i0 = ((int64_t)tile << kTileShift) + 128 * (int)threadIdx.y + 2 * lane;
+  const uint32_t i0u = (uint32_t)i0;
+  q &= 1;
+  const __amdgpu_buffer_rsrc_t h2 = pop_rsrc(par(sc.h, q), (uint32_t)(Q7 * 2 * S));
+  const __amdgpu_buffer_rsrc_t pr = pop_rsrc(par(sc.phi, q), (uint32_t)(4 * S));
+  float *__restrict__ phi_next = par(sc.phi, q ^ 1);
+  const uint8_t *__restrict__ tag = sc.tag;
+  int box[6];
+#pragma unroll
+  for (int j = 0; j < 6; j++) box[j] = sc.ctrl->box_fused[j];
+  bool fl[2];
+  uint32_t nf[2];
+#pragma unroll
+  for (int J = 0; J < 2; J++) {
+    const int64_t i = i0 + J;
+    int x, y, z;
+    cell_xyz(g.nx, g.nxny, i, x, y, z);
+    const bool near = in_box(box, x, y, z);
+    fl[J] = i < g.n && interior(g, x, y, z) && (!near || ldb(tag, (uint32_t)i) == TAG_FLUID);
+    nf[J] = 0;
+    if (fl[J]) {
+      if (near) {
+#pragma unroll
+        for (int k = 1; k < Q7; k++)
+          nf[J] |= (uint32_t)(ldb(tag, (uint32_t)(i - dir_off(g, k))) != TAG_FLUID) << k;
+      } else {
+        nf[J] = wall_mask(g, x, y, z) & 0x7eu;
+      }
+    }
+  }
+  // Every load in flight before any is used.
+  uint32_t hr[Q7], hx[Q7], ps[Q7][2];
+  static_for<0, Q7>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    const uint32_t off = (uint32_t)dir_off(g, k);
+    pull_pair<PullOdd<PC, k>::v, false>(h2, k * S32, i0u, off, hr[k], hx[k]);
+    ps[k][0] = __builtin_amdgcn_raw_buffer_load_b32(pr, (int)(4u * (i0u - off)), 0, 0);
+    ps[k][1] = __builtin_amdgcn_raw_buffer_load_b32(pr, (int)(4u * (i0u + 1u - off)), 0, 0);
+  });
+  static_for<0, Q7>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    hr[k] = pull_join<PullOdd<PC, k>::v>(hr[k], hx[k]);
+  });
+  // Halfway bounce-back: the own opposite pair where a source is not fluid.
+  const uint32_t nfa = nf[0] | nf[1];
+  if (nfa) {
+    uint32_t oh[Q7];
+#pragma unroll
+    for (int k = 1; k < Q7; k++)
+      oh[k] = ((nfa >> k) & 1u) ? ld_pair<false>(h2, opp7(k) * S32 + i0u) : 0u;
+#pragma unroll
+    for (int k = 1; k < Q7; k++) {
+      const uint32_t bm = (((nf[0] >> k) & 1u) ? 0x0000ffffu : 0u) |
+                          (((nf[1] >> k) & 1u) ? 0xffff0000u : 0u);
+      hr[k] = (hr[k] & ~bm) | (oh[k] & bm);
+    }
+  }
+  float out[2];
+#pragma unroll
+  for (int J = 0; J < 2; J++) {
+    // phi of the cell: the rest direction's source is the cell itself.
+    const float pc = __builtin_bit_cast(float, ps[0][J]);
+    if (fl[J]) {
+      float e[Q7], phib[Q7], hfull[Q7];
+#pragma unroll
+      for (int k = 0; k < Q7; k++) {
+        const bool bounce = (nf[J] >> k) & 1u;
+        e[k] = C::dv((uint16_t)(hr[k] >> (16 * J)));
+        phib[k] = bounce ? pc : __builtin_bit_cast(float, ps[k][J]);
+      }
+      out[J] = phase_sum_dev(e, phib, hfull);
+    } else {
+      out[J] = pc;
+    }
+  }
+  if (i0 + 1 < g.n) {
+    stb(phi_next, i0u * 4u, out[0]);
+    stb(phi_next, i0u * 4u + 4u, out[1]);
+  } else if (i0 < g.n) {
+    stb(phi_next, i0u * 4u, out[0]);
+  }
+}
+template <int KIND>
+__device__ __forceinline__ void phase_stream_pair(const Scene &sc, int lb, int q) {
+  int tile;
+  if (!map_tile(sc.tm.T, sc.tm.T8, lb, tile)) return;
+  const int pc = parity_class(sc.g.nx, sc.g.ny);
+  if (pc == 0) phase_stream_pair_body<KIND, 0>(sc, tile, q);
+  else if (pc == 1) phase_stream_pair_body<KIND, 1>(sc, tile, q);
+  else phase_stream_pair_body<KIND, 2>(sc, tile, q);
+}
+template <int KIND>
+__global__ __launch_bounds__(128) void km_phase_stream2(const Scene *__restrict__ scenes,
+                                                        const int2 *__restrict__ tiles, int q) {
+  const int2 t = tiles[blockIdx.x];
+  phase_stream_pair<KIND>(scenes[t.x], t.y, q);
+}
+template <int KIND>
+__global__ __launch_bounds__(128) void k1_phase_stream2(const Scene sc, int q) {
+  phase_stream_pair<KIND>(sc, blockIdx.x, q);
+}
+
 // ---------------------------------------------------------------- storage conversion
 
 // Float32 populations (the reference's f and h) to a 16-bit codec, and back.
