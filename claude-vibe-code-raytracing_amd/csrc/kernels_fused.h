@@ -238,37 +238,51 @@ __device__ __forceinline__ void collide_hydro_cell(const Derived &d, R sumF, R p
 // is formed once per pair.
 
 // Hydro collide in deviation form (collideHydro, src/sim.js:430-459): fin(k)
-// gives f_k - w_k of the cell and out(k, v) receives the post-collision
+// gives d_k = f_k - w_k of the cell and out(k, v) receives the post-collision
 // deviation of direction k; rhof is rho as stored (sum f).
+//
+// The BGK step with Guo forcing, d_k + (1/tau)(w_k (rho - 1 + rho_phase Q_k)
+// - d_k) + A w_k ((e_k - u).F / cs^2 + (e_k.u)(e_k.F) / cs^4), with
+// Q_k = 3 e.u + 4.5 (e.u)^2 - 1.5 u^2, is rearranged as
+//   (1 - 1/tau) d_k + w_k (c0 + c1 e.u + c2 (e.u)^2 + A e.F + 3 A (e.u)(e.F))
+// with per-cell c0 = (rho_phase (-1.5 u^2) + rho - 1) / tau - A u.F,
+// c1 = 3 rho_phase / tau, c2 = 4.5 rho_phase / tau.  For an opposite pair
+// (e, -e) the bracket is sym +- anti with sym = c0 + e.u (c2 e.u + 3 A e.F) and
+// anti = c1 e.u + A e.F, and the weight w_k is folded into the constants of
+// its class (faces, edges, corners): about 10 operations per pair instead of 18.
 template <int K, typename FIn, typename FOut>
-__device__ __forceinline__ void hydro_dev_pair(float sdev, float rhoPhase, float nu2h, float uF,
-                                               float ux, float uy, float uz, float fx, float fy,
-                                               float fz, float it, float A, FIn &fin, FOut &out) {
-  // K: the first direction of a pair; K + 1 is its opposite.
+__device__ __forceinline__ void hydro_dev_pair(float c0, float c1, float c2, float om, float ux,
+                                               float uy, float uz, float ax, float ay, float az,
+                                               FIn &fin, FOut &out) {
+  // K: the first direction of a pair; K + 1 is its opposite.  c0..c2 and
+  // (ax, ay, az) = A F carry the pair's weight.
   constexpr int ex = EX27[K], ey = EY27[K], ez = EZ27[K];
-  const float wk = w27(K);  // == w27(K + 1)
-  const float Awk = A * wk;
   const float eu = edot<ex, ey, ez>(ux, uy, uz);
-  const float ekF = edot<ex, ey, ez>(fx, fy, fz);
-#pragma unroll
-  for (int j = 0; j < 2; j++) {
-    const float e = j ? -eu : eu, eF = j ? -ekF : ekF;
-    const float din_k = fin(K + j);
-    const float Qi = fmaf(e, fmaf(float(INV_2CS4_27), e, float(INV_CS2_27)), nu2h);
-    const float diff = fmaf(wk, fmaf(rhoPhase, Qi, sdev), -din_k);
-    const float o = fmaf(it, diff, din_k);
-    const float S = fmaf(eF, fmaf(float(INV_CS2_27), e, 1.0f), -uF);
-    out(K + j, fmaf(Awk, S, o));
+  const float aF = edot<ex, ey, ez>(ax, ay, az);
+  const float sym = fmaf(eu, fmaf(c2, eu, 3.0f * aF), c0);
+  const float anti = fmaf(c1, eu, aF);
+  out(K, fmaf(om, fin(K), sym + anti));
+  out(K + 1, fmaf(om, fin(K + 1), sym - anti));
+}
+// Pairs K, K + 2, ... below KEND, all of one weight.
+template <int K, int KEND, typename FIn, typename FOut>
+__device__ __forceinline__ void hydro_dev_pairs(float c0, float c1, float c2, float om, float ux,
+                                                float uy, float uz, float ax, float ay, float az,
+                                                FIn &fin, FOut &out) {
+  if constexpr (K < KEND) {
+    static_assert(w27(K) == w27(KEND - 1), "one weight class");
+    hydro_dev_pair<K>(c0, c1, c2, om, ux, uy, uz, ax, ay, az, fin, out);
+    hydro_dev_pairs<K + 2, KEND>(c0, c1, c2, om, ux, uy, uz, ax, ay, az, fin, out);
   }
 }
-template <int K, typename FIn, typename FOut>
-__device__ __forceinline__ void hydro_dev_pairs(float sdev, float rhoPhase, float nu2h, float uF,
-                                                float ux, float uy, float uz, float fx, float fy,
-                                                float fz, float it, float A, FIn &fin, FOut &out) {
-  if constexpr (K < Q27) {
-    hydro_dev_pair<K>(sdev, rhoPhase, nu2h, uF, ux, uy, uz, fx, fy, fz, it, A, fin, out);
-    hydro_dev_pairs<K + 2>(sdev, rhoPhase, nu2h, uF, ux, uy, uz, fx, fy, fz, it, A, fin, out);
-  }
+// The pairs of directions K .. KEND - 1 with the class weight folded in.
+template <int K, int KEND, typename FIn, typename FOut>
+__device__ __forceinline__ void hydro_dev_class(float c0, float c1, float c2, float om, float ux,
+                                                float uy, float uz, float afx, float afy,
+                                                float afz, FIn &fin, FOut &out) {
+  constexpr float w = w27(K);
+  hydro_dev_pairs<K, KEND>(w * c0, w * c1, w * c2, om, ux, uy, uz, w * afx, w * afy, w * afz,
+                           fin, out);
 }
 template <typename FIn, typename FOut>
 __device__ __forceinline__ void collide_hydro_dev(const DerivedF &d, float rhof, float p, float lap,
@@ -282,16 +296,17 @@ __device__ __forceinline__ void collide_hydro_dev(const DerivedF &d, float rhof,
   const float fz = bz + mu * ggz;
   const float nu2h = -1.5f * (ux * ux + uy * uy + uz * uz);
   const float uF = ux * fx + uy * fy + uz * fz;
-  const float sdev = rhof - 1.0f;
   const float it = d.invTau;
-  // k = 0: e = 0, so Q = -1.5 u^2 and the forcing term is -A w0 u.F.
-  {
-    const float w0 = w27(0);
-    const float din0 = fin(0);
-    const float diff = fmaf(w0, fmaf(rhoPhase, nu2h, sdev), -din0);
-    out(0, fmaf(d.A * w0, -uF, fmaf(it, diff, din0)));
-  }
-  hydro_dev_pairs<1>(sdev, rhoPhase, nu2h, uF, ux, uy, uz, fx, fy, fz, it, d.A, fin, out);
+  const float c0 = fmaf(it, fmaf(rhoPhase, nu2h, rhof - 1.0f), -d.A * uF);
+  const float itr = it * rhoPhase;
+  const float c1 = float(INV_CS2_27) * itr, c2 = float(INV_2CS4_27) * itr;
+  const float om = 1.0f - it;
+  const float afx = d.A * fx, afy = d.A * fy, afz = d.A * fz;
+  // k = 0: e = 0, so the bracket is c0.
+  out(0, fmaf(om, fin(0), w27(0) * c0));
+  hydro_dev_class<1, 7>(c0, c1, c2, om, ux, uy, uz, afx, afy, afz, fin, out);
+  hydro_dev_class<7, 19>(c0, c1, c2, om, ux, uy, uz, afx, afy, afz, fin, out);
+  hydro_dev_class<19, 27>(c0, c1, c2, om, ux, uy, uz, afx, afy, afz, fin, out);
 }
 
 // Phase collide in deviation form (collidePhase, src/sim.js:341-360): hin and
