@@ -1734,6 +1734,18 @@ __global__ __launch_bounds__(256) void k_check_partials(int64_t n, const uint8_t
   }
 }
 
+// ---------------------------------------------------------------- control blocks
+
+// Copies the control block of every scene of a set (block j: scene j) into
+// one array, so the host reads the set's blocks back with one transfer.
+static_assert(sizeof(Ctrl) % 4 == 0, "control block copied in words");
+__global__ __launch_bounds__(64) void km_gather_ctrl(const Scene *__restrict__ scenes,
+                                                     Ctrl *__restrict__ out) {
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(scenes[blockIdx.x].ctrl);
+  uint32_t *dst = reinterpret_cast<uint32_t *>(out + blockIdx.x);
+  for (int w = threadIdx.x; w < (int)(sizeof(Ctrl) / 4); w += 64) dst[w] = src[w];
+}
+
 // ---------------------------------------------------------------- body
 
 // Body update after stream(t) of step q, one 512-thread workgroup per scene:
