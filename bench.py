@@ -47,6 +47,8 @@ HBM_PEAK_GBS = 8000.0
 # rho/u written + tag.
 KERNEL_BYTES_PER_LU = {"f32": 281, "f16": 145, "bf16": 145}
 STEP_BYTES_PER_LU = {"f32": 297, "f16": 161, "bf16": 161}
+# Populations alone (27 + 7 planes read and written): the pattern probe's bytes.
+PATTERN_BYTES_PER_LU = {"f32": 272, "f16": 136, "bf16": 136}
 TRAFFIC_JSON = {"f32": os.path.join(ROOT, "profiles", "stream_collide_256cube_traffic.json"),
                 "f16": os.path.join(ROOT, "profiles", "stream_collide_256cube_f16_traffic.json")}
 
@@ -297,6 +299,7 @@ def main():
     if args.workload == "serve":
         return bench_serve(args)
     n = args.size
+    pattern_gbps = lbm.pull_bandwidth(n, n, n, args.storage, 10)
     sim, b, params, yw, sph = build_scene(n, args.seed)
     sim.set_storage(args.storage)
     print(f"[bench] scene {n}^3 built; warmup {args.warmup}", file=sys.stderr, flush=True)
@@ -361,6 +364,11 @@ def main():
         "whole_step_hbm_frac": round(STEP_BYTES_PER_LU[args.storage] * mlups * 1e6 /
                                      (HBM_PEAK_GBS * 1e9), 4),
         "copy_probe_gbps": round(copy_gbps, 1),
+        # The stream kernel's population pattern alone (34 shifted planes pulled
+        # and stored, no arithmetic): the ceiling its memory pattern allows.
+        "pattern_probe": {"gbps": round(pattern_gbps, 1),
+                          "bytes_per_lu": PATTERN_BYTES_PER_LU[args.storage],
+                          "kernel_frac": round(achieved / pattern_gbps, 4) if achieved else None},
         "kernel_ms": {k: {"total": round(v[0], 3), "launches": v[1]} for k, v in kt.items()},
         "finite": finite,
         "sphere": {"cx": b.cx, "cy": b.cy, "cz": b.cz, "vx": b.vx, "vy": b.vy, "vz": b.vz},

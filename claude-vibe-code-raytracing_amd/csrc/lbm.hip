@@ -1080,6 +1080,103 @@ __global__ __launch_bounds__(256) void k_copy_probe(const float4 *__restrict__ a
   if (i < n4) b[i] = a[i];
 }
 
+// The population pattern of the fused stream kernels with no arithmetic: every
+// one of the 27 + 7 planes pulled from its shifted source (the own element
+// where the source is outside the grid) and stored, non-temporally, in the
+// kernels' tiles and order.  One float32 cell per lane, or two 16-bit cells per
+// lane in aligned 4-byte pairs (an odd offset straddles two pairs).
+__global__ __launch_bounds__(256) void k_pull_probe32(const float *__restrict__ src,
+                                                      float *__restrict__ dst, Geom g,
+                                                      TileMap tm) {
+  int tile;
+  if (!map_tile(tm.T, tm.T8, blockIdx.x, tile)) return;
+  const int64_t i = ((int64_t)tile << kTileShift) + threadIdx.x;
+  if (i >= g.n) return;
+  const uint32_t bo = (uint32_t)i * 4u;
+#pragma unroll
+  for (int k = 0; k < Q27 + Q7; k++) {
+    const int kk = k < Q27 ? k : k - Q27;
+    const int64_t off = (int64_t)EZ27[kk] * g.nxny + (int64_t)EY27[kk] * g.nx + EX27[kk];
+    const bool in = i >= off && i - off < g.n;
+    const float *plane = src + (int64_t)k * g.S;
+    stn(dst + (int64_t)k * g.S, bo, ldb(plane, in ? bo - (uint32_t)off * 4u : bo));
+  }
+}
+__global__ __launch_bounds__(128) void k_pull_probe16(const uint16_t *__restrict__ src,
+                                                      uint16_t *__restrict__ dst, Geom g,
+                                                      TileMap tm) {
+  int tile;
+  if (!map_tile(tm.T, tm.T8, blockIdx.x, tile)) return;
+  const uint32_t i0 = (uint32_t)(((int64_t)tile << kTileShift) + 128 * threadIdx.y + 2 * threadIdx.x);
+  const uint32_t S32 = (uint32_t)g.S;
+  const __amdgpu_buffer_rsrc_t r = pop_rsrc(src, (uint32_t)((Q27 + Q7) * 2 * g.S));
+  const __amdgpu_buffer_rsrc_t w = pop_rsrc(dst, (uint32_t)((Q27 + Q7) * 2 * g.S));
+#pragma unroll
+  for (int k = 0; k < Q27 + Q7; k++) {
+    const int kk = k < Q27 ? k : k - Q27;
+    const uint32_t off =
+        (uint32_t)((int64_t)EZ27[kk] * g.nxny + (int64_t)EY27[kk] * g.nx + EX27[kk]);
+    const uint32_t e = (uint32_t)k * S32 + i0 - off;
+    uint32_t v;
+    if (off & 1u) {
+      const uint32_t lo = ld_pair<true>(r, e - 1u), hi = ld_pair<true>(r, e + 1u);
+      v = (lo >> 16) | (hi << 16);
+    } else {
+      v = ld_pair<true>(r, e);
+    }
+    if (i0 < (uint32_t)g.n) st_pair(w, (uint32_t)k * S32 + i0, v);
+  }
+}
+
+extern "C" int lbm_probe_pull_bandwidth(int nx, int ny, int nz, int storage, int reps,
+                                        double *gbps) {
+  if (nx < 3 || ny < 3 || nz < 3 || reps < 1 || !gbps) return LBM_ERR_ARG;
+  if (storage != LBM_STORAGE_F32 && storage != LBM_STORAGE_F16 && storage != LBM_STORAGE_BF16)
+    return LBM_ERR_ARG;
+  Geom g;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.nxny = nx * ny;
+  g.n = (int64_t)g.nxny * nz;
+  g.S = (g.n + 63) / 64 * 64;
+  const int elem = storage == LBM_STORAGE_F32 ? 4 : 2;
+  const int64_t bytes = (int64_t)(Q27 + Q7) * g.S * elem;
+  if (elem == 4 ? g.S >= kF32FusedMaxCells : bytes >= ((int64_t)1 << 32)) return LBM_ERR_ARG;
+  const TileMap tm = make_tile_map(nx, ny, nz);
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = LBM_ERR_HIP;
+  float ms = 0;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+    rc = LBM_ERR_OOM;
+    goto done;
+  }
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) goto done;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) goto done;
+  if (hipMemsetAsync(a, 0, bytes, st) != hipSuccess) goto done;
+  for (int r = -1; r < reps; r++) {  // r = -1: warm-up
+    if (r == 0) hipEventRecord(e0, st);
+    if (elem == 4)
+      k_pull_probe32<<<tm.T8, 256, 0, st>>>((const float *)a, (float *)b, g, tm);
+    else
+      k_pull_probe16<<<tm.T8, dim3(64, 2, 1), 0, st>>>((const uint16_t *)a, (uint16_t *)b, g, tm);
+  }
+  hipEventRecord(e1, st);
+  if (hipEventSynchronize(e1) != hipSuccess) goto done;
+  hipEventElapsedTime(&ms, e0, e1);
+  *gbps = 2.0 * (Q27 + Q7) * elem * (double)g.n * reps / (ms * 1e-3) / 1e9;
+  rc = LBM_OK;
+done:
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (st) hipStreamDestroy(st);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  return rc;
+}
+
 extern "C" int lbm_probe_copy_bandwidth(int64_t bytes, int reps, double *gbps) {
   if (bytes < 16 || reps < 1 || !gbps) return LBM_ERR_ARG;
   const int64_t n4 = bytes / 16;

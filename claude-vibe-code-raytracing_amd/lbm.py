@@ -167,6 +167,8 @@ def lib():
         "lbm_set_stream_lanes": (C.c_int, [P, C.c_int]),
         "lbm_version": (C.c_char_p, []),
         "lbm_probe_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
+        "lbm_probe_pull_bandwidth": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                               C.POINTER(C.c_double)]),
         "lbm_state_bytes": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int]),
         "lbm_server_create": (C.c_int, [C.c_int64, C.c_int, C.c_int64, C.POINTER(P)]),
         "lbm_server_destroy": (None, [P]),
@@ -207,6 +209,17 @@ def make_sphere(nx, ny, nz, cx=float("nan"), cy=float("nan"), cz=float("nan"),
     b = Sphere()
     lib().lbm_sphere_init(C.byref(b), nx, ny, nz, cx, cy, cz, r, density)
     return b
+
+
+def pull_bandwidth(nx, ny, nz, storage="f32", reps=10):
+    """Measured read+write GB/s of the stream kernels' population pattern alone
+    (34 shifted planes pulled and stored, no arithmetic) on an nx x ny x nz grid."""
+    g = C.c_double(0)
+    rc = lib().lbm_probe_pull_bandwidth(int(nx), int(ny), int(nz), STORAGE[storage], int(reps),
+                                        C.byref(g))
+    if rc != 0:
+        raise LbmError(f"lbm_probe_pull_bandwidth = {rc}")
+    return g.value
 
 
 def copy_bandwidth(nbytes=1 << 31, reps=10):
