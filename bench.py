@@ -91,24 +91,30 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(n, seed, steps):
-    """The oracle (single thread) on the same scene: MLUPS over `steps` steps."""
+def cpu_baseline(n, seed, steps, parallel=False):
+    """The oracle on the same scene: MLUPS over `steps` steps, on one thread, or
+    with parallel=True the OpenMP build (same results) on OMP_NUM_THREADS
+    threads (default: every core)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py as O
+    bits = "32_omp" if parallel else 32
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() if parallel else 1
     params, yw, sph = scenegen.config1(n, n, n, seed)
-    s = O.OSim(n, n, n)
+    s = O.OSim(n, n, n, bits=bits)
     s.set_params(**params)
     s.init_waterline_map(yw)
-    b = O.sphere(n, n, n, sph["cx"], sph["cy"], sph["cz"], sph["r"], sph["density"])
+    b = O.sphere(n, n, n, sph["cx"], sph["cy"], sph["cz"], sph["r"], sph["density"], bits=bits)
     b.vx, b.vy, b.vz = sph["vx"], sph["vy"], sph["vz"]
     s.coupled(b, 1)  # first step includes the initial dead-cell sweep
     t0 = time.perf_counter()
     s.coupled(b, steps)
     dt = time.perf_counter() - t0
-    return {"value": round(n ** 3 * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": 1,
+    how = (f"the OpenMP build (z-planes of each per-cell pass in parallel) on {threads} threads"
+           if parallel else "1 thread")
+    return {"value": round(n ** 3 * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": threads,
             "kind": "port",
             "sample": f"{n}^3 config-1 scene, {steps} coupled steps after 1 untimed, oracle "
-                      f"(C restatement of sim.js/solid.js, FP64 arithmetic) on 1 thread of "
+                      f"(C restatement of sim.js/solid.js, FP64 arithmetic), {how} of "
                       f"{cpu_model()}"}
 
 
@@ -267,6 +273,8 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-par-steps", type=int, default=10,
+                    help="steps of the multi-threaded oracle baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="only run the plain copy kernel (counter calibration)")
@@ -378,6 +386,8 @@ def main():
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(n, args.seed, args.cpu_steps)
+        print("[bench] cpu baseline, all cores", file=sys.stderr, flush=True)
+        out["cpu_baseline_parallel"] = cpu_baseline(n, args.seed, args.cpu_par_steps, True)
     print(json.dumps(out), flush=True)
     return 0
 

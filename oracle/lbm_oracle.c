@@ -10,8 +10,20 @@
 #include "lbm_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Built with -fopenmp (liboracle32_omp.so, a CPU timing baseline) the per-cell
+ * passes run their z-planes in parallel: each cell writes only its own
+ * elements, and the one order-dependent sum (momentum exchange into the body)
+ * is applied after the pass in cell order, so every thread count gives the
+ * sequential result bit for bit. */
+#ifdef _OPENMP
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define OMP_FOR
+#endif
 
 enum { Q27 = 27, Q7 = 7, TAG_FLUID = 0, TAG_SOLID = 1, TAG_WALL = 2 };
 
@@ -200,6 +212,7 @@ void oracle_seed_hydrostatic(osim *s) {
 void oracle_compute_phi_gradients(osim *s) {
   const int nx = s->nx, ny = s->ny, nz = s->nz;
   const ostore_t *phi = s->phi;
+  OMP_FOR
   for (int z = 0; z < nz; z++) {
     const int zm = z > 0 ? z - 1 : z, zp = z < nz - 1 ? z + 1 : z;
     for (int y = 0; y < ny; y++) {
@@ -228,9 +241,10 @@ void oracle_retag_cells(osim *s, const osphere *solid) {
   const int nx = s->nx, ny = s->ny, nz = s->nz;
   memcpy(s->tagPrev, s->tag, (size_t)s->n);
   memcpy(s->phiPrev, s->phi, (size_t)s->n * sizeof(ostore_t));
-  double tmp[3] = {0, 0, 0};
   const int nxny = nx * ny;
+  OMP_FOR
   for (int z = 0; z < nz; z++) {
+    double tmp[3] = {0, 0, 0};
     const int zOnWall = z == 0 || z == nz - 1;
     const int zBase = z * nxny;
     for (int y = 0; y < ny; y++) {
@@ -322,6 +336,7 @@ void oracle_collide_phase(osim *s) {
   const double guoHalf = 1.0 - 0.5 * invTau;
   const double sharpC = 4.0 * Mphi * guoHalf / W;
   const double sharpFac = INV_CS2_7;
+  OMP_FOR
   for (int z = 1; z < nz - 1; z++) {
     const int zBase = z * nxny;
     for (int y = 1; y < ny - 1; y++) {
@@ -356,6 +371,7 @@ void oracle_collide_phase(osim *s) {
 void oracle_stream_phase(osim *s) {
   const int nx = s->nx, ny = s->ny, nz = s->nz;
   const int nxny = nx * ny;
+  OMP_FOR
   for (int z = 1; z < nz - 1; z++) {
     const int zBase = z * nxny;
     for (int y = 1; y < ny - 1; y++) {
@@ -385,6 +401,7 @@ void oracle_collide_hydro(osim *s) {
   const double dRho = rhoL - rhoG;
   const double A = (1 - 0.5 * invTau) * INV_CS2_27;
   const int nxny = nx * ny;
+  OMP_FOR
   for (int z = 1; z < nz - 1; z++) {
     const int zBase = z * nxny;
     for (int y = 1; y < ny - 1; y++) {
@@ -422,8 +439,38 @@ void oracle_collide_hydro(osim *s) {
   }
 }
 
+/* Momentum-exchange impulses of one z-plane, in cell order. */
+typedef struct {
+  int x, y, z;
+  double px, py, pz;
+} oimpulse;
+typedef struct {
+  oimpulse *v;
+  size_t n, cap;
+} oimplist;
+static void imp_push(oimplist *l, int x, int y, int z, double px, double py, double pz) {
+  if (l->n == l->cap) {
+    l->cap = l->cap ? 2 * l->cap : 256;
+    l->v = (oimpulse *)realloc(l->v, l->cap * sizeof *l->v);
+    if (!l->v) {
+      fprintf(stderr, "oracle: out of memory\n");
+      abort();
+    }
+  }
+  oimpulse *e = &l->v[l->n++];
+  e->x = x;
+  e->y = y;
+  e->z = z;
+  e->px = px;
+  e->py = py;
+  e->pz = pz;
+}
+
 /* sim.js:484-582 streamHydroAndMacro: pull stream, halfway bounce-back with the
- * moving-wall term, momentum exchange into the body, then rho / phi / u. */
+ * moving-wall term, momentum exchange into the body, then rho / phi / u.  The
+ * impulses are recorded per plane and applied after the cell loop in the
+ * reference's cell order (they only add to the body's force and torque, which
+ * the loop does not read). */
 void oracle_stream_hydro_and_macro(osim *s, osphere *solid) {
   const int nx = s->nx, ny = s->ny, nz = s->nz;
   const double sigma = s->sigma, gravity = s->gravity;
@@ -432,6 +479,12 @@ void oracle_stream_hydro_and_macro(osim *s, osphere *solid) {
   const double beta = 12 * sigma / W;
   const double kappa = 1.5 * sigma * W;
   const double dRho = rhoL - rhoG;
+  oimplist *imp = (oimplist *)calloc((size_t)nz, sizeof *imp);
+  if (!imp) {
+    fprintf(stderr, "oracle: out of memory\n");
+    abort();
+  }
+  OMP_FOR
   for (int z = 1; z < nz - 1; z++) {
     const int zBase = z * nxny;
     for (int y = 1; y < ny - 1; y++) {
@@ -461,7 +514,7 @@ void oracle_stream_hydro_and_macro(osim *s, osphere *solid) {
             if (sTag == TAG_SOLID && solid) {
               const double m = fopp + fk;
               const int sx = x - (int)ekx, sy = y - (int)eky, sz = z - (int)ekz;
-              oracle_sphere_apply_impulse_at_cell(solid, sx, sy, sz, -m * ekx, -m * eky, -m * ekz);
+              imp_push(&imp[z], sx, sy, sz, -m * ekx, -m * eky, -m * ekz);
             }
           }
           s->f[i27 + k] = (ostore_t)fk;
@@ -498,6 +551,14 @@ void oracle_stream_hydro_and_macro(osim *s, osphere *solid) {
       }
     }
   }
+  for (int z = 0; z < nz; z++) {
+    for (size_t j = 0; j < imp[z].n; j++) {
+      const oimpulse *e = &imp[z].v[j];
+      oracle_sphere_apply_impulse_at_cell(solid, e->x, e->y, e->z, e->px, e->py, e->pz);
+    }
+    free(imp[z].v);
+  }
+  free(imp);
   memset(s->bodyFx, 0, (size_t)s->n * sizeof(ostore_t));
   memset(s->bodyFy, 0, (size_t)s->n * sizeof(ostore_t));
   memset(s->bodyFz, 0, (size_t)s->n * sizeof(ostore_t));

@@ -50,13 +50,16 @@ def build():
 
 
 def lib(bits=32):
+    """bits: 32 (float32 storage, the reference's), 64 (FP64 storage), or
+    "32_omp" (the float32 build with its per-cell passes in OpenMP threads,
+    same results)."""
     if bits in _LIBS:
         return _LIBS[bits]
     path = os.path.join(BUILD, f"liboracle{bits}.so")
     if not os.path.exists(path):
         build()
     L = C.CDLL(path)
-    store = C.c_float if bits == 32 else C.c_double
+    store = C.c_double if bits == 64 else C.c_float
     S = _osim_struct(store)
     SP = C.POINTER(S)
     B = C.POINTER(OSphere)
@@ -105,7 +108,7 @@ class OSim:
         self.ptr = self.L.oracle_sim_create(nx, ny, nz)
         self.s = self.ptr.contents
         self.nx, self.ny, self.nz, self.n = nx, ny, nz, nx * ny * nz
-        dt = np.float32 if bits == 32 else np.float64
+        dt = np.float64 if bits == 64 else np.float32
         for k in FIELDS:
             cnt = self.n * POP.get(k, 1)
             arr = np.ctypeslib.as_array(getattr(self.s, k), shape=(cnt,))
