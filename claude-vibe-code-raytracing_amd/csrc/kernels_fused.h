@@ -1521,7 +1521,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
 // baselines, so phi(t+1) is the one-cell kernel's bit for bit.  Loads go
 // through buffer resources: an offset outside a buffer (a wall cell's source
 // before the grid) reads 0 and is never used.
-template <int KIND, int PC>
+template <int KIND, int PC, bool NTL>
 __device__ __forceinline__ void phase_stream_pair_body(const Scene &sc, int tile, int q) {
   using C = Codec<KIND>;
   static_assert(KIND != ST_F32, "16-bit codecs only");
@@ -1564,7 +1564,7 @@ __device__ __forceinline__ void phase_stream_pair_body(const Scene &sc, int tile
   static_for<0, Q7>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     const uint32_t off = (uint32_t)dir_off(g, k);
-    pull_pair<PullOdd<PC, k>::v, false>(h2, k * S32, i0u, off, hr[k], hx[k]);
+    pull_pair<PullOdd<PC, k>::v, NTL>(h2, k * S32, i0u, off, hr[k], hx[k]);
     ps[k][0] = __builtin_amdgcn_raw_buffer_load_b32(pr, (int)(4u * (i0u - off)), 0, 0);
     ps[k][1] = __builtin_amdgcn_raw_buffer_load_b32(pr, (int)(4u * (i0u + 1u - off)), 0, 0);
   });
@@ -1611,24 +1611,25 @@ __device__ __forceinline__ void phase_stream_pair_body(const Scene &sc, int tile
     stb(phi_next, i0u * 4u, out[0]);
   }
 }
-template <int KIND>
+// NTL: non-temporal h loads, as for the pair stream kernel (one-scene launches).
+template <int KIND, bool NTL>
 __device__ __forceinline__ void phase_stream_pair(const Scene &sc, int lb, int q) {
   int tile;
   if (!map_tile(sc.tm.T, sc.tm.T8, lb, tile)) return;
   const int pc = parity_class(sc.g.nx, sc.g.ny);
-  if (pc == 0) phase_stream_pair_body<KIND, 0>(sc, tile, q);
-  else if (pc == 1) phase_stream_pair_body<KIND, 1>(sc, tile, q);
-  else phase_stream_pair_body<KIND, 2>(sc, tile, q);
+  if (pc == 0) phase_stream_pair_body<KIND, 0, NTL>(sc, tile, q);
+  else if (pc == 1) phase_stream_pair_body<KIND, 1, NTL>(sc, tile, q);
+  else phase_stream_pair_body<KIND, 2, NTL>(sc, tile, q);
 }
 template <int KIND>
 __global__ __launch_bounds__(128) void km_phase_stream2(const Scene *__restrict__ scenes,
                                                         const int2 *__restrict__ tiles, int q) {
   const int2 t = tiles[blockIdx.x];
-  phase_stream_pair<KIND>(scenes[t.x], t.y, q);
+  phase_stream_pair<KIND, false>(scenes[t.x], t.y, q);
 }
 template <int KIND>
 __global__ __launch_bounds__(128) void k1_phase_stream2(const Scene sc, int q) {
-  phase_stream_pair<KIND>(sc, blockIdx.x, q);
+  phase_stream_pair<KIND, true>(sc, blockIdx.x, q);
 }
 
 // ---------------------------------------------------------------- storage conversion
