@@ -868,6 +868,11 @@ extern "C" int lbm_set_storage(lbm_sim *s, int kind) {
     for (int b = 0; b < 2; b++) {
       HIP_TRY(s, hipMalloc(&s->cf[b], S * 27 * 2));
       HIP_TRY(s, hipMalloc(&s->ch[b], S * 7 * 2));
+      // The fused step never writes the populations of cells that are not
+      // fluid: clear them so that what reads back there is defined (a zero
+      // deviation) rather than whatever the allocation held.
+      HIP_TRY(s, hipMemsetAsync(s->cf[b], 0, S * 27 * 2, s->stream));
+      HIP_TRY(s, hipMemsetAsync(s->ch[b], 0, S * 7 * 2, s->stream));
     }
   }
   s->storage = kind;
