@@ -1,0 +1,104 @@
+"""The 256^3 scene the benchmark steps (perturbed waterline at 0.6 of the
+height, an r = 24 sphere of density 3 moving at 0.02 cells per step), at its
+full size.
+
+float32 populations: the fused step against the oracle's OpenMP build (the
+same results as its sequential build) over calls of 1, 4, 10 and 25 steps --
+after every call the solid mask and the previous mask bit for bit, the fresh
+and dead cell lists equal to the ones the oracle's masks define (interior
+cells, ascending index, as handleFreshCells and absorbDeadCells visit them,
+sim.js:257-310 and solid.js), phi/rho/u within 1e-5 of the field maximum over
+interior fluid cells and the sphere centre within 1e-4 cells.
+
+FP16 populations: the two-cells-per-lane stream kernel against the one-cell
+kernel, every field, the populations of every interior fluid cell and the
+sphere bit for bit after two calls."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from gpu_util import announce, interior_fluid, pop_fluid, rel_err
+from lbm_py import lbm, scenegen
+
+pytestmark = pytest.mark.gpu
+
+N = 256
+SEED = 1
+TAG_FLUID, TAG_SOLID = 0, 1
+
+
+def gpu_scene(storage):
+    params, yw, sph = scenegen.config1(N, N, N, SEED)
+    sim = lbm.Sim(N, N, N)
+    sim.set_params(**params)
+    sim.init_waterline_map(yw)
+    sim.set_storage(storage)
+    b = lbm.make_sphere(N, N, N, sph["cx"], sph["cy"], sph["cz"], sph["r"], sph["density"])
+    b.vx, b.vy, b.vz = sph["vx"], sph["vy"], sph["vz"]
+    return sim, b
+
+
+def oracle_scene():
+    params, yw, sph = scenegen.config1(N, N, N, SEED)
+    s = O.OSim(N, N, N, bits="32_omp")
+    s.set_params(**params)
+    s.init_waterline_map(yw)
+    b = O.sphere(N, N, N, sph["cx"], sph["cy"], sph["cz"], sph["r"], sph["density"],
+                 bits="32_omp")
+    b.vx, b.vy, b.vz = sph["vx"], sph["vy"], sph["vz"]
+    return s, b
+
+
+def interior():
+    m = np.zeros((N, N, N), dtype=bool)
+    m[1:-1, 1:-1, 1:-1] = True
+    return m.ravel()
+
+
+def test_fused_vs_oracle_256cube():
+    announce("fused float32 vs oracle (OpenMP build), 256^3 config-1 scene, 40 steps")
+    so, bo = oracle_scene()
+    sg, bg = gpu_scene("f32")
+    inner = interior()
+    flips = [0, 0]
+    for steps in (1, 4, 10, 25):
+        so.coupled(bo, steps)
+        sg.coupled(bg, steps)
+        announce("after a call of", steps, "steps")
+        assert max(abs(bo.cx - bg.cx), abs(bo.cy - bg.cy), abs(bo.cz - bg.cz)) <= 1e-4
+        tag, tag_prev = sg.get("tag"), sg.get("tagPrev")
+        assert np.array_equal(tag, so.tag)
+        assert np.array_equal(tag_prev, so.tagPrev)
+        fresh = np.flatnonzero(inner & (so.tag == TAG_FLUID) & (so.tagPrev == TAG_SOLID))
+        dead = np.flatnonzero(inner & (so.tag == TAG_SOLID) & (so.tagPrev == TAG_FLUID))
+        assert np.array_equal(np.asarray(sg.fresh_cells(), np.int64), fresh)
+        assert np.array_equal(np.asarray(sg.dead_cells(), np.int64), dead)
+        flips[0] += fresh.size
+        flips[1] += dead.size
+        mask = interior_fluid(so.tag, N, N, N)
+        for k in ("phi", "rho", "ux", "uy", "uz"):
+            e = rel_err(sg.get(k), getattr(so, k), mask)
+            assert e <= 1e-5, (steps, k, e)
+    # The sphere moved across cells: both lists were exercised.
+    assert flips[0] > 0 and flips[1] > 0, flips
+    sg.close()
+
+
+def test_pair_kernel_equals_one_cell_kernel_256cube():
+    announce("FP16 pair vs one-cell stream kernel, 256^3 config-1 scene, 9 steps")
+    out = []
+    for lanes in (1, 2):
+        sim, b = gpu_scene("f16")
+        sim.set_stream_lanes(lanes)
+        sim.coupled(b, 5)
+        sim.coupled(b, 4)
+        fields = {k: sim.get(k) for k in ("phi", "phiPrev", "rho", "ux", "uy", "uz", "tag",
+                                          "tagPrev")}
+        fluid = interior_fluid(fields["tag"], N, N, N)
+        fields["f"] = pop_fluid(sim.get("f"), 27, fluid)
+        fields["h"] = pop_fluid(sim.get("h"), 7, fluid)
+        out.append((fields, b.as_list()))
+        sim.close()
+    for k in out[0][0]:
+        assert np.array_equal(out[0][0][k].view(np.uint8), out[1][0][k].view(np.uint8)), k
+    assert out[0][1] == out[1][1]
