@@ -1089,7 +1089,11 @@ __global__ __launch_bounds__(256) void k_copy_probe(const float4 *__restrict__ a
 // one of the 27 + 7 planes pulled from its shifted source (the own element
 // where the source is outside the grid) and stored, non-temporally, in the
 // kernels' tiles and order.  One float32 cell per lane, or two 16-bit cells per
-// lane in aligned 4-byte pairs (an odd offset straddles two pairs).
+// lane in aligned 4-byte pairs (an odd offset straddles two pairs).  As in the
+// kernels, every pull is issued before the first store: buffer loads are not
+// moved past buffer stores by the compiler, and a probe that stores each
+// direction after its load runs the 16-bit pattern at 4.2 instead of 5.0 TB/s
+// (256^3, MI355X).
 __global__ __launch_bounds__(256) void k_pull_probe32(const float *__restrict__ src,
                                                       float *__restrict__ dst, Geom g,
                                                       TileMap tm) {
@@ -1098,14 +1102,16 @@ __global__ __launch_bounds__(256) void k_pull_probe32(const float *__restrict__ 
   const int64_t i = ((int64_t)tile << kTileShift) + threadIdx.x;
   if (i >= g.n) return;
   const uint32_t bo = (uint32_t)i * 4u;
+  float v[Q27 + Q7];
 #pragma unroll
   for (int k = 0; k < Q27 + Q7; k++) {
     const int kk = k < Q27 ? k : k - Q27;
     const int64_t off = (int64_t)EZ27[kk] * g.nxny + (int64_t)EY27[kk] * g.nx + EX27[kk];
     const bool in = i >= off && i - off < g.n;
-    const float *plane = src + (int64_t)k * g.S;
-    stn(dst + (int64_t)k * g.S, bo, ldb(plane, in ? bo - (uint32_t)off * 4u : bo));
+    v[k] = ldb(src + (int64_t)k * g.S, in ? bo - (uint32_t)off * 4u : bo);
   }
+#pragma unroll
+  for (int k = 0; k < Q27 + Q7; k++) stn(dst + (int64_t)k * g.S, bo, v[k]);
 }
 __global__ __launch_bounds__(128) void k_pull_probe16(const uint16_t *__restrict__ src,
                                                       uint16_t *__restrict__ dst, Geom g,
@@ -1116,20 +1122,28 @@ __global__ __launch_bounds__(128) void k_pull_probe16(const uint16_t *__restrict
   const uint32_t S32 = (uint32_t)g.S;
   const __amdgpu_buffer_rsrc_t r = pop_rsrc(src, (uint32_t)((Q27 + Q7) * 2 * g.S));
   const __amdgpu_buffer_rsrc_t w = pop_rsrc(dst, (uint32_t)((Q27 + Q7) * 2 * g.S));
+  uint32_t lo[Q27 + Q7], hi[Q27 + Q7];
 #pragma unroll
   for (int k = 0; k < Q27 + Q7; k++) {
     const int kk = k < Q27 ? k : k - Q27;
     const uint32_t off =
         (uint32_t)((int64_t)EZ27[kk] * g.nxny + (int64_t)EY27[kk] * g.nx + EX27[kk]);
     const uint32_t e = (uint32_t)k * S32 + i0 - off;
-    uint32_t v;
     if (off & 1u) {
-      const uint32_t lo = ld_pair<true>(r, e - 1u), hi = ld_pair<true>(r, e + 1u);
-      v = (lo >> 16) | (hi << 16);
+      lo[k] = ld_pair<true>(r, e - 1u);
+      hi[k] = ld_pair<true>(r, e + 1u);
     } else {
-      v = ld_pair<true>(r, e);
+      lo[k] = ld_pair<true>(r, e);
+      hi[k] = 0u;
     }
-    if (i0 < (uint32_t)g.n) st_pair(w, (uint32_t)k * S32 + i0, v);
+  }
+  if (i0 >= (uint32_t)g.n) return;
+#pragma unroll
+  for (int k = 0; k < Q27 + Q7; k++) {
+    const int kk = k < Q27 ? k : k - Q27;
+    const uint32_t off =
+        (uint32_t)((int64_t)EZ27[kk] * g.nxny + (int64_t)EY27[kk] * g.nx + EX27[kk]);
+    st_pair(w, (uint32_t)k * S32 + i0, (off & 1u) ? (lo[k] >> 16) | (hi[k] << 16) : lo[k]);
   }
 }
 
