@@ -415,8 +415,8 @@ int lbm_set_stream_lanes(lbm_sim *sim, int cells_per_lane);
 int lbm_probe_copy_bandwidth(int64_t bytes, int reps, double *gbps);
 /* Measured rate of the fused stream kernels' population pattern alone: the
  * 27 + 7 planes of an nx x ny x nz grid pulled from their shifted sources and
- * stored, in the kernels' tiles, order and cache policy, with no arithmetic
- * (storage: LBM_STORAGE_F32 one cell per lane, or a 16-bit form two cells per
+ * stored, in the kernels' tiles, order and cache policy (every pull of a lane
+ * issued before its first store), with no arithmetic (storage: LBM_STORAGE_F32 one cell per lane, or a 16-bit form two cells per
  * lane); `reps` timed passes after one warm-up, read+write rate in GB/s over
  * 34 planes x element bytes x cells.  No reference counterpart: the ceiling the
  * stream kernels' memory pattern allows, beside the roofline. */
