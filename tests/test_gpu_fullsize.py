@@ -10,9 +10,9 @@ cells, ascending index, as handleFreshCells and absorbDeadCells visit them,
 sim.js:257-310 and solid.js), phi/rho/u within 1e-5 of the field maximum over
 interior fluid cells and the sphere centre within 1e-4 cells.
 
-FP16 populations: the two-cells-per-lane stream kernel against the one-cell
-kernel, every field, the populations of every interior fluid cell and the
-sphere bit for bit after two calls."""
+FP16 and BF16 populations: the two-cells-per-lane stream kernel against the
+one-cell kernel, every field, the populations of every interior fluid cell and
+the sphere bit for bit after two calls."""
 import numpy as np
 import pytest
 
@@ -84,11 +84,12 @@ def test_fused_vs_oracle_256cube():
     sg.close()
 
 
-def test_pair_kernel_equals_one_cell_kernel_256cube():
-    announce("FP16 pair vs one-cell stream kernel, 256^3 config-1 scene, 9 steps")
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_pair_kernel_equals_one_cell_kernel_256cube(storage):
+    announce(storage, "pair vs one-cell stream kernel, 256^3 config-1 scene, 9 steps")
     out = []
     for lanes in (1, 2):
-        sim, b = gpu_scene("f16")
+        sim, b = gpu_scene(storage)
         sim.set_stream_lanes(lanes)
         sim.coupled(b, 5)
         sim.coupled(b, 4)
