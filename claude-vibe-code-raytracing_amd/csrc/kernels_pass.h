@@ -563,17 +563,44 @@ __global__ __launch_bounds__(256) void k_dead_impulses(const int32_t *__restrict
   }
 }
 
-// packVolume (src/sim.js:586-600).
+// packVolume (src/sim.js:586-600) for one cell: R = (clamp(phi, 0, 1) * 255) | 0,
+// G = 255 on solid cells; the two bytes as one little-endian 16-bit value.
+__device__ __forceinline__ uint16_t pack_rg8(float phi, uint8_t tag) {
+  double p = phi;
+  if (p < 0) p = 0;
+  else if (p > 1) p = 1;
+  return (uint16_t)((uint8_t)(int32_t)(p * 255) | (tag == TAG_SOLID ? 0xff00u : 0u));
+}
+
 __global__ __launch_bounds__(256) void k_pack(int64_t n, const float *__restrict__ phi,
                                               const uint8_t *__restrict__ tag,
                                               uint8_t *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double p = phi[i];
-  if (p < 0) p = 0;
-  else if (p > 1) p = 1;
-  out[2 * i] = (uint8_t)(int32_t)(p * 255);
-  out[2 * i + 1] = tag[i] == TAG_SOLID ? 255 : 0;
+  const uint16_t v = pack_rg8(phi[i], tag[i]);
+  out[2 * i] = (uint8_t)v;
+  out[2 * i + 1] = (uint8_t)(v >> 8);
+}
+
+// The frames of many scenes packed by one launch: grid row j packs the cells
+// of job j into out + off[j] (an even byte offset), its blocks striding over
+// them.
+constexpr int kPackJobs = 32;
+struct PackJobs {
+  const float *phi[kPackJobs];
+  const uint8_t *tag[kPackJobs];
+  int64_t n[kPackJobs];
+  int64_t off[kPackJobs];
+};
+__global__ __launch_bounds__(256) void k_pack_jobs(const PackJobs jobs, uint8_t *__restrict__ out) {
+  const int j = blockIdx.y;
+  const int64_t n = jobs.n[j];
+  const float *__restrict__ phi = jobs.phi[j];
+  const uint8_t *__restrict__ tag = jobs.tag[j];
+  uint16_t *__restrict__ o = reinterpret_cast<uint16_t *>(out + jobs.off[j]);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    o[i] = pack_rg8(phi[i], tag[i]);
 }
 
 // Layout transposes between the reference's array-of-structures (q values per
