@@ -51,15 +51,16 @@ def build():
 
 def lib(bits=32):
     """bits: 32 (float32 storage, the reference's), 64 (FP64 storage), or
-    "32_omp" (the float32 build with its per-cell passes in OpenMP threads,
-    same results)."""
+    "32_omp" / "64_omp" (those builds with their per-cell passes in OpenMP
+    threads, same results)."""
     if bits in _LIBS:
         return _LIBS[bits]
     path = os.path.join(BUILD, f"liboracle{bits}.so")
     if not os.path.exists(path):
         build()
     L = C.CDLL(path)
-    store = C.c_double if bits == 64 else C.c_float
+    fp64 = bits in (64, "64_omp")
+    store = C.c_double if fp64 else C.c_float
     S = _osim_struct(store)
     SP = C.POINTER(S)
     B = C.POINTER(OSphere)
@@ -108,7 +109,7 @@ class OSim:
         self.ptr = self.L.oracle_sim_create(nx, ny, nz)
         self.s = self.ptr.contents
         self.nx, self.ny, self.nz, self.n = nx, ny, nz, nx * ny * nz
-        dt = np.float64 if bits == 64 else np.float32
+        dt = np.float64 if bits in (64, "64_omp") else np.float32
         for k in FIELDS:
             cnt = self.n * POP.get(k, 1)
             arr = np.ctypeslib.as_array(getattr(self.s, k), shape=(cnt,))

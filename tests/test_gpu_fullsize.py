@@ -12,12 +12,17 @@ interior fluid cells and the sphere centre within 1e-4 cells.
 
 FP16 and BF16 populations: the two-cells-per-lane stream kernel against the
 one-cell kernel, every field, the populations of every interior fluid cell and
-the sphere bit for bit after two calls."""
+the sphere bit for bit after two calls.
+
+Every storage form against the oracle with FP64 storage and arithmetic (its
+OpenMP build) after 200 steps: phi within the bounds the 2000-step check of the
+batched scenes uses, sphere centre within 0.05 cells over the trajectory."""
 import numpy as np
 import pytest
 
 import oracle_py as O
 from gpu_util import announce, interior_fluid, pop_fluid, rel_err
+from test_gpu_precision import PHI_MAX, errors
 from lbm_py import lbm, scenegen
 
 pytestmark = pytest.mark.gpu
@@ -38,13 +43,13 @@ def gpu_scene(storage):
     return sim, b
 
 
-def oracle_scene():
+def oracle_scene(bits="32_omp"):
     params, yw, sph = scenegen.config1(N, N, N, SEED)
-    s = O.OSim(N, N, N, bits="32_omp")
+    s = O.OSim(N, N, N, bits=bits)
     s.set_params(**params)
     s.init_waterline_map(yw)
     b = O.sphere(N, N, N, sph["cx"], sph["cy"], sph["cz"], sph["r"], sph["density"],
-                 bits="32_omp")
+                 bits=bits)
     b.vx, b.vy, b.vz = sph["vx"], sph["vy"], sph["vz"]
     return s, b
 
@@ -103,3 +108,36 @@ def test_pair_kernel_equals_one_cell_kernel_256cube(storage):
     for k in out[0][0]:
         assert np.array_equal(out[0][0][k].view(np.uint8), out[1][0][k].view(np.uint8)), k
     assert out[0][1] == out[1][1]
+
+
+PREC_STEPS, PREC_SAMPLE = 200, 50
+
+
+@pytest.fixture(scope="module")
+def fp64_256cube():
+    announce("fp64 oracle (OpenMP build), 256^3 config-1 scene,", PREC_STEPS, "steps")
+    o, b = oracle_scene("64_omp")
+    traj = []
+    for _ in range(PREC_STEPS // PREC_SAMPLE):
+        o.coupled(b, PREC_SAMPLE)
+        traj.append((b.cx, b.cy, b.cz))
+    return dict(phi=o.phi.copy(), ux=o.ux.copy(), uy=o.uy.copy(), uz=o.uz.copy(),
+                tag=o.tag.copy(), traj=np.array(traj))
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_precision_vs_fp64_oracle_256cube(storage, fp64_256cube):
+    announce("precision", storage, "256^3 config-1 scene,", PREC_STEPS, "steps")
+    sim, b = gpu_scene(storage)
+    traj = []
+    for _ in range(PREC_STEPS // PREC_SAMPLE):
+        sim.coupled(b, PREC_SAMPLE)
+        traj.append((b.cx, b.cy, b.cz))
+    g = dict(phi=sim.get("phi"), ux=sim.get("ux"), uy=sim.get("uy"), uz=sim.get("uz"),
+             tag=sim.get("tag"), traj=np.array(traj))
+    sim.close()
+    e = errors(g, fp64_256cube)
+    announce(storage, e)
+    assert np.isfinite(g["phi"]).all()
+    assert e["phi_max"] <= PHI_MAX[storage], e
+    assert e["sphere_max"] <= 0.05, e

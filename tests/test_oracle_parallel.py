@@ -1,5 +1,6 @@
-"""The OpenMP build of the oracle (a multi-core CPU baseline) computes exactly
-what the sequential build computes: every field and the sphere state after
+"""The OpenMP builds of the oracle (float32 storage: a multi-core CPU baseline;
+FP64 storage: precision checks at full size) compute exactly what the
+sequential builds compute: every field and the sphere state after
 40 coupled steps of a 40x28x40 scene whose sphere is moved by the momentum it
 exchanges (below the acceleration cap's reach, so the order of the force sum
 shows in the trajectory: replaying the planes' impulses in reverse order
@@ -19,7 +20,7 @@ import numpy as np
 sys.path.insert(0, sys.argv[1])
 import oracle_py as O
 bits = sys.argv[2]
-bits = 32 if bits == "32" else bits
+bits = {"32": 32, "64": 64}.get(bits, bits)
 params = dict(tau=0.6, tauPhi=0.58, sigma=0.01, gravity=1e-4, rhoL=1.0, rhoG=0.1, W=4.0)
 s = O.OSim(40, 28, 40, bits=bits)
 s.set_params(**params)
@@ -41,9 +42,10 @@ def run(bits, threads):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def test_parallel_oracle_equals_sequential():
-    ref = run("32", 1)
+@pytest.mark.parametrize("bits", ["32", "64"])
+def test_parallel_oracle_equals_sequential(bits):
+    ref = run(bits, 1)
     for threads in (2, 5):
-        got = run("32_omp", threads)
+        got = run(bits + "_omp", threads)
         for k in ref:
             assert got[k] == ref[k], (threads, k)
