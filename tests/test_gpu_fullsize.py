@@ -15,8 +15,11 @@ one-cell kernel, every field, the populations of every interior fluid cell and
 the sphere bit for bit after two calls.
 
 Every storage form against the oracle with FP64 storage and arithmetic (its
-OpenMP build) after 200 steps: phi within the bounds the 2000-step check of the
-batched scenes uses, sphere centre within 0.05 cells over the trajectory."""
+OpenMP build) after 200 steps (LBM_FULLSIZE_PREC_STEPS sets another count):
+phi within the bounds the 2000-step check of the batched scenes uses, sphere
+centre within 0.05 cells over the trajectory."""
+import os
+
 import numpy as np
 import pytest
 
@@ -110,7 +113,10 @@ def test_pair_kernel_equals_one_cell_kernel_256cube(storage):
     assert out[0][1] == out[1][1]
 
 
-PREC_STEPS, PREC_SAMPLE = 200, 50
+# LBM_FULLSIZE_PREC_STEPS (a multiple of 50) lengthens the run, e.g. to the
+# 2000 steps of the precision target.
+PREC_STEPS = int(os.environ.get("LBM_FULLSIZE_PREC_STEPS", "200"))
+PREC_SAMPLE = 50
 
 
 @pytest.fixture(scope="module")
@@ -118,9 +124,10 @@ def fp64_256cube():
     announce("fp64 oracle (OpenMP build), 256^3 config-1 scene,", PREC_STEPS, "steps")
     o, b = oracle_scene("64_omp")
     traj = []
-    for _ in range(PREC_STEPS // PREC_SAMPLE):
+    for k in range(PREC_STEPS // PREC_SAMPLE):
         o.coupled(b, PREC_SAMPLE)
         traj.append((b.cx, b.cy, b.cz))
+        announce("fp64 oracle step", (k + 1) * PREC_SAMPLE)
     return dict(phi=o.phi.copy(), ux=o.ux.copy(), uy=o.uy.copy(), uz=o.uz.copy(),
                 tag=o.tag.copy(), traj=np.array(traj))
 
@@ -137,7 +144,7 @@ def test_precision_vs_fp64_oracle_256cube(storage, fp64_256cube):
              tag=sim.get("tag"), traj=np.array(traj))
     sim.close()
     e = errors(g, fp64_256cube)
-    announce(storage, e)
+    announce(storage, PREC_STEPS, "steps", e)
     assert np.isfinite(g["phi"]).all()
     assert e["phi_max"] <= PHI_MAX[storage], e
     assert e["sphere_max"] <= 0.05, e
