@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -134,6 +135,32 @@ struct lbm_sim {
 
 static std::string g_create_error;
 
+// Device and pinned-host allocation and release.  Each holds g_alloc_mu, which a
+// launch-graph capture also holds from its start to the graph's instantiation,
+// as do lbm_create and lbm_destroy (streams created and destroyed): an
+// allocation, a release or a stream's creation or destruction on another thread
+// (any of which may synchronise the device) then never meets a stream that is
+// being captured.
+static std::recursive_mutex g_alloc_mu;
+template <typename T>
+static hipError_t dev_malloc(T **p, size_t bytes) {
+  std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
+  return hipMalloc((void **)p, bytes);
+}
+static hipError_t dev_free(void *p) {
+  std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
+  return hipFree(p);
+}
+template <typename T>
+static hipError_t host_malloc(T **p, size_t bytes, unsigned flags) {
+  std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
+  return hipHostMalloc((void **)p, bytes, flags);
+}
+static hipError_t host_free(void *p) {
+  std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
+  return hipHostFree(p);
+}
+
 // ------------------------------------------------------------------ helpers
 
 #define HIP_TRY(sim, expr)                                                              \
@@ -188,10 +215,10 @@ static float *PHIPREV(lbm_sim *s) { return s->phibuf[1 - s->phicur]; }
 
 static int ensure_scratch(lbm_sim *s, size_t bytes) {
   if (s->scratch_bytes >= bytes) return LBM_OK;
-  if (s->scratch) hipFree(s->scratch);
+  if (s->scratch) dev_free(s->scratch);
   s->scratch = nullptr;
   s->scratch_bytes = 0;
-  HIP_TRY(s, hipMalloc(&s->scratch, bytes));
+  HIP_TRY(s, dev_malloc(&s->scratch, bytes));
   s->scratch_bytes = bytes;
   return LBM_OK;
 }
@@ -228,23 +255,23 @@ static int ensure_f32(lbm_sim *s, hipStream_t st) {
 static int ensure_links(lbm_sim *s, int64_t cap) {
   if (cap > INT32_MAX) return fail(s, LBM_ERR_CAPACITY, "link list capacity exceeds 2^31");
   if (s->link_cap >= cap) return LBM_OK;
-  hipFree(s->link_key);
-  hipFree(s->link_m);
+  dev_free(s->link_key);
+  dev_free(s->link_m);
   s->link_key = nullptr;
   s->link_m = nullptr;
   s->link_cap = 0;
-  HIP_TRY(s, hipMalloc(&s->link_key, cap * sizeof(int64_t)));
-  HIP_TRY(s, hipMalloc(&s->link_m, cap * sizeof(double)));
+  HIP_TRY(s, dev_malloc(&s->link_key, cap * sizeof(int64_t)));
+  HIP_TRY(s, dev_malloc(&s->link_m, cap * sizeof(double)));
   s->link_cap = (int)cap;
   return LBM_OK;
 }
 
 static int ensure_imp(lbm_sim *s, int64_t cap) {
   if (s->imp_cap >= cap) return LBM_OK;
-  hipFree(s->imp);
+  dev_free(s->imp);
   s->imp = nullptr;
   s->imp_cap = 0;
-  HIP_TRY(s, hipMalloc(&s->imp, std::max<int64_t>(cap, 1) * 3 * sizeof(double)));
+  HIP_TRY(s, dev_malloc(&s->imp, std::max<int64_t>(cap, 1) * 3 * sizeof(double)));
   s->imp_cap = (int)cap;
   return LBM_OK;
 }
@@ -322,6 +349,7 @@ extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
     g_create_error = "no HIP device available";
     return LBM_ERR_NODEVICE;
   }
+  std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
   lbm_sim *s = new lbm_sim();
   s->nx = nx;
   s->ny = ny;
@@ -342,7 +370,7 @@ extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
   hipError_t e = hipSuccess;
 #define ALLOC(ptr, bytes)                                        \
   if (e == hipSuccess) {                                         \
-    e = hipMalloc((void **)&(ptr), (bytes));                     \
+    e = dev_malloc((void **)&(ptr), (bytes));                     \
   }
   ALLOC(s->fbuf[0], S * 27 * sizeof(float));
   ALLOC(s->fbuf[1], S * 27 * sizeof(float));
@@ -371,7 +399,7 @@ extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
   ALLOC(s->level_scratch, (size_t)cap * sizeof(int));
   ALLOC(s->ctrl, sizeof(Ctrl));
 #undef ALLOC
-  if (e == hipSuccess) e = hipHostMalloc((void **)&s->ctrl_host, sizeof(Ctrl), hipHostMallocDefault);
+  if (e == hipSuccess) e = host_malloc((void **)&s->ctrl_host, sizeof(Ctrl), hipHostMallocDefault);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e == hipSuccess) {
     // Zero the whole state so reads before an init are defined.
@@ -400,6 +428,7 @@ extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
 
 extern "C" void lbm_destroy(lbm_sim *s) {
   if (!s) return;
+  std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
   if (s->stream) hipStreamSynchronize(s->stream);
   collect_timing(s->timer);
   void *ptrs[] = {s->fbuf[0], s->fbuf[1], s->hbuf[0], s->hbuf[1], s->phibuf[0], s->phibuf[1],
@@ -409,8 +438,8 @@ extern "C" void lbm_destroy(lbm_sim *s) {
                   s->mask_dev, s->mus_dev, s->partials, s->cf[0], s->cf[1], s->ch[0],
                   s->ch[1]};
   for (void *p : ptrs)
-    if (p) hipFree(p);
-  if (s->ctrl_host) hipHostFree(s->ctrl_host);
+    if (p) dev_free(p);
+  if (s->ctrl_host) host_free(s->ctrl_host);
   s->graphs.clear();
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;
@@ -541,8 +570,8 @@ extern "C" int lbm_retag_cells_mask(lbm_sim *s, const uint8_t *mask, const float
   if (!mask || !us) return fail(s, LBM_ERR_ARG, "mask/us is NULL");
   const int64_t n = s->g.n;
   if (!s->mask_dev) {
-    HIP_TRY(s, hipMalloc(&s->mask_dev, n));
-    HIP_TRY(s, hipMalloc(&s->mus_dev, n * 3 * sizeof(float)));
+    HIP_TRY(s, dev_malloc(&s->mask_dev, n));
+    HIP_TRY(s, dev_malloc(&s->mus_dev, n * 3 * sizeof(float)));
   }
   HIP_TRY(s, hipMemcpyAsync(s->mask_dev, mask, n, hipMemcpyHostToDevice, s->stream));
   HIP_TRY(s, hipMemcpyAsync(s->mus_dev, us, n * 3 * sizeof(float), hipMemcpyHostToDevice, s->stream));
@@ -866,8 +895,8 @@ extern "C" int lbm_set_storage(lbm_sim *s, int kind) {
   if (kind != LBM_STORAGE_F32 && !s->cf[0]) {
     const int64_t S = s->g.S;
     for (int b = 0; b < 2; b++) {
-      HIP_TRY(s, hipMalloc(&s->cf[b], S * 27 * 2));
-      HIP_TRY(s, hipMalloc(&s->ch[b], S * 7 * 2));
+      HIP_TRY(s, dev_malloc(&s->cf[b], S * 27 * 2));
+      HIP_TRY(s, dev_malloc(&s->ch[b], S * 7 * 2));
       // The fused step never writes the populations of cells that are not
       // fluid: clear them so that what reads back there is defined (a zero
       // deviation) rather than whatever the allocation held.
@@ -1168,7 +1197,7 @@ extern "C" int lbm_probe_pull_bandwidth(int nx, int ny, int nz, int storage, int
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = LBM_ERR_HIP;
   float ms = 0;
-  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+  if (dev_malloc(&a, bytes) != hipSuccess || dev_malloc(&b, bytes) != hipSuccess) {
     rc = LBM_ERR_OOM;
     goto done;
   }
@@ -1191,8 +1220,8 @@ done:
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);
   if (st) hipStreamDestroy(st);
-  if (a) hipFree(a);
-  if (b) hipFree(b);
+  if (a) dev_free(a);
+  if (b) dev_free(b);
   return rc;
 }
 
@@ -1204,7 +1233,7 @@ extern "C" int lbm_probe_copy_bandwidth(int64_t bytes, int reps, double *gbps) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = LBM_ERR_HIP;
   float ms = 0;
-  if (hipMalloc(&a, n4 * 16) != hipSuccess || hipMalloc(&b, n4 * 16) != hipSuccess) {
+  if (dev_malloc(&a, n4 * 16) != hipSuccess || dev_malloc(&b, n4 * 16) != hipSuccess) {
     rc = LBM_ERR_OOM;
     goto done;
   }
@@ -1226,8 +1255,8 @@ done:
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);
   if (st) hipStreamDestroy(st);
-  if (a) hipFree(a);
-  if (b) hipFree(b);
+  if (a) dev_free(a);
+  if (b) dev_free(b);
   return rc;
 }
 

@@ -47,6 +47,7 @@ export class Sim {
     this._mirror = {};   // field -> typed array handed to the caller
     this._shadow = {};   // field -> copy last exchanged with the device
     this._params = new Float64Array(8);
+    this._busy = false;  // a runAsync call has not resolved yet
 
     const nxny = nx * ny;
     this.neighOffset27 = new Int32Array(Q27);
@@ -79,7 +80,16 @@ export class Sim {
   rhoFromPhi(p) { return this.rhoG + p * (this.rhoL - this.rhoG); }
 
   // Frees the device state now instead of at garbage collection.
-  dispose() { native.destroy(this._h); }
+  dispose() {
+    this._idle();
+    native.destroy(this._h);
+  }
+
+  // Every use of the Sim (and of a RigidSphere it steps) throws while a
+  // runAsync call is in flight: the worker thread owns the device state then.
+  _idle() {
+    if (this._busy) throw new Error('Sim is in use by runAsync until its promise settles');
+  }
 
   // ---------------------------------------------------------------- plumbing
   _alloc(name) {
@@ -88,6 +98,7 @@ export class Sim {
   }
 
   _field(name) {
+    this._idle();
     let m = this._mirror[name];
     if (!m) {
       m = this._alloc(name);
@@ -123,6 +134,7 @@ export class Sim {
   }
 
   _call(fn) {
+    this._idle();
     this._push();
     const r = fn();
     this._pull();
@@ -216,16 +228,26 @@ export class Sim {
   // mirrors refreshed.  The Sim must not be used until then.  Generic solids
   // step synchronously (their callbacks run on this thread).
   runAsync(solid, nsteps) {
+    this._idle();
     if (solid != null && !(solid instanceof RigidSphere)) {
       this.run(solid, nsteps);
       return Promise.resolve();
     }
+    if (solid) solid._idle();
     this._push();
-    return native.stepAsync(this._h, solid ? solid.state : null, nsteps).then(() => this._pull());
+    const p = native.stepAsync(this._h, solid ? solid.state : null, nsteps);
+    this._busy = true;
+    if (solid) solid._busy = true;
+    const done = () => {
+      this._busy = false;
+      if (solid) solid._busy = false;
+    };
+    return p.then(() => { done(); this._pull(); }, (e) => { done(); throw e; });
   }
 
   // Opt-in state check (not on the hot path): { finite, maxU } over fluid cells.
   checkFields() {
+    this._idle();
     this._push();
     const [finite, maxU] = native.checkFields(this._h);
     return { finite, maxU };
@@ -235,11 +257,12 @@ export class Sim {
   _absorbDead(solid) { this._call(() => native.absorbDead(this._h, solid.state)); }
 
   // Cells that turned fluid / solid at the last retag, ascending.
-  freshCells() { return native.freshCells(this._h); }
-  deadCells() { return native.deadCells(this._h); }
+  freshCells() { this._idle(); return native.freshCells(this._h); }
+  deadCells() { this._idle(); return native.deadCells(this._h); }
 
   // Sim.packVolume(out, _) (src/sim.js:586-600)
   packVolume(out, _solidContains) {
+    this._idle();
     this._push();
     native.pack(this._h, out);
   }

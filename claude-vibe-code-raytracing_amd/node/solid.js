@@ -18,12 +18,18 @@ export class RigidSphere {
   // new RigidSphere(nx, ny, nz, opts) (src/solid.js:9-37)
   constructor(nx, ny, nz, opts = {}) {
     this.state = new Float64Array(FIELDS.length);
+    this._busy = false;  // stepped by a Sim.runAsync that has not settled
     native.sphereInit(this.state, nx, ny, nz, opt(opts, 'cx'), opt(opts, 'cy'), opt(opts, 'cz'),
       opt(opts, 'r'), opt(opts, 'density'));
   }
 
+  // Writes throw while a Sim.runAsync steps this sphere on a worker thread.
+  _idle() {
+    if (this._busy) throw new Error('RigidSphere is in use by Sim.runAsync until its promise settles');
+  }
+
   // src/solid.js:39-47
-  recomputeMass() { native.sphereRecomputeMass(this.state); }
+  recomputeMass() { this._idle(); native.sphereRecomputeMass(this.state); }
   setRadius(r) { this.r = r; this.recomputeMass(); }
   setDensity(d) { this.density = d; this.recomputeMass(); }
 
@@ -31,20 +37,24 @@ export class RigidSphere {
   contains(x, y, z) { return native.sphereContains(this.state, x, y, z); }
   surfaceVelocity(x, y, z, out) { return native.sphereSurfaceVelocity(this.state, x, y, z, out); }
   applyImpulseAtCell(x, y, z, px, py, pz) {
+    this._idle();
     native.sphereApplyImpulse(this.state, x, y, z, px, py, pz);
   }
 
   // src/solid.js:87-101: dead cells of sim's last retag, in index order.
-  absorbDeadCells(sim) { sim._absorbDead(this); }
+  absorbDeadCells(sim) { this._idle(); sim._absorbDead(this); }
 
   // src/solid.js:104-210
-  step(dt, gravity) { native.sphereStep(this.state, dt, gravity); }
+  step(dt, gravity) { this._idle(); native.sphereStep(this.state, dt, gravity); }
 }
 
 FIELDS.forEach((k, idx) => {
   Object.defineProperty(RigidSphere.prototype, k, {
     get() { return k === 'dragging' ? this.state[idx] !== 0 : this.state[idx]; },
-    set(v) { this.state[idx] = k === 'dragging' ? (v ? 1 : 0) : v; },
+    set(v) {
+      this._idle();
+      this.state[idx] = k === 'dragging' ? (v ? 1 : 0) : v;
+    },
     enumerable: true,
   });
 });

@@ -1,6 +1,7 @@
 // Sim.runAsync (lbm_step on a worker thread, a Promise) gives the same state as
-// Sim.run, and the event loop keeps running while it steps; checkFields reports
-// a finite state and |u| under the reference's cap.
+// Sim.run, and the event loop keeps running while it steps; until it settles the
+// Sim and its sphere refuse every use; checkFields reports a finite state and
+// |u| under the reference's cap.
 import { Sim } from '../../claude-vibe-code-raytracing_amd/node/sim.js';
 import { RigidSphere } from '../../claude-vibe-code-raytracing_amd/node/solid.js';
 
@@ -20,7 +21,16 @@ async function main() {
   a.sim.run(a.solid, 60);
   let ticks = 0;
   const timer = setInterval(() => { ticks++; }, 0);
-  await b.sim.runAsync(b.solid, 30);
+  const pending = b.sim.runAsync(b.solid, 30);
+  // While the worker thread steps them, the Sim and its sphere refuse use.
+  let refused = 0;
+  for (const use of [() => b.sim.phi, () => b.sim.run(b.solid, 1), () => b.sim.runAsync(b.solid, 1),
+    () => b.sim.packVolume(new Uint8Array(2 * b.sim.n)), () => { b.solid.vy = 0; },
+    () => b.solid.step(1.0, 1e-5)]) {
+    try { use(); } catch (e) { refused++; }
+  }
+  if (refused !== 6) fail(`busy guard refused ${refused} of 6 uses`);
+  await pending;
   await b.sim.runAsync(b.solid, 30);
   clearInterval(timer);
   for (const k of ['phi', 'rho', 'ux', 'uy', 'uz', 'tag']) {

@@ -10,9 +10,9 @@ cells, ascending index, as handleFreshCells and absorbDeadCells visit them,
 sim.js:257-310 and solid.js), phi/rho/u within 1e-5 of the field maximum over
 interior fluid cells and the sphere centre within 1e-4 cells.
 
-FP16 and BF16 populations: the two-cells-per-lane stream kernel against the
-one-cell kernel, every field, the populations of every interior fluid cell and
-the sphere bit for bit after two calls.
+FP16 and BF16 populations, at 256^3 and 384^3: the two-cells-per-lane stream
+kernel against the one-cell kernel, every field, the populations of every
+interior fluid cell and the sphere bit for bit after two calls.
 
 Every storage form against the oracle with FP64 storage and arithmetic (its
 OpenMP build) after 200 steps (LBM_FULLSIZE_PREC_STEPS sets another count):
@@ -35,13 +35,13 @@ SEED = 1
 TAG_FLUID, TAG_SOLID = 0, 1
 
 
-def gpu_scene(storage):
-    params, yw, sph = scenegen.config1(N, N, N, SEED)
-    sim = lbm.Sim(N, N, N)
+def gpu_scene(storage, n=N):
+    params, yw, sph = scenegen.config1(n, n, n, SEED)
+    sim = lbm.Sim(n, n, n)
     sim.set_params(**params)
     sim.init_waterline_map(yw)
     sim.set_storage(storage)
-    b = lbm.make_sphere(N, N, N, sph["cx"], sph["cy"], sph["cz"], sph["r"], sph["density"])
+    b = lbm.make_sphere(n, n, n, sph["cx"], sph["cy"], sph["cz"], sph["r"], sph["density"])
     b.vx, b.vy, b.vz = sph["vx"], sph["vy"], sph["vz"]
     return sim, b
 
@@ -92,18 +92,21 @@ def test_fused_vs_oracle_256cube():
     sg.close()
 
 
+@pytest.mark.parametrize("n", [256, 384])
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
-def test_pair_kernel_equals_one_cell_kernel_256cube(storage):
-    announce(storage, "pair vs one-cell stream kernel, 256^3 config-1 scene, 9 steps")
+def test_pair_kernel_equals_one_cell_kernel_fullsize(storage, n):
+    """At 384^3 the 16-bit f buffer spans about 3 GB: the pair kernel's 32-bit
+    buffer offsets then pass 2^31 bytes."""
+    announce(storage, "pair vs one-cell stream kernel,", f"{n}^3 config-1 scene, 9 steps")
     out = []
     for lanes in (1, 2):
-        sim, b = gpu_scene(storage)
+        sim, b = gpu_scene(storage, n)
         sim.set_stream_lanes(lanes)
         sim.coupled(b, 5)
         sim.coupled(b, 4)
         fields = {k: sim.get(k) for k in ("phi", "phiPrev", "rho", "ux", "uy", "uz", "tag",
                                           "tagPrev")}
-        fluid = interior_fluid(fields["tag"], N, N, N)
+        fluid = interior_fluid(fields["tag"], n, n, n)
         fields["f"] = pop_fluid(sim.get("f"), 27, fluid)
         fields["h"] = pop_fluid(sim.get("h"), 7, fluid)
         out.append((fields, b.as_list()))

@@ -112,3 +112,47 @@ def test_autotune_steps_like_plain_calls():
         bt.close()
     for sim, _ in sims + refs:
         sim.close()
+
+
+def test_capture_beside_create_and_destroy_on_another_thread():
+    """One thread steps fresh simulations whose first call captures a graph
+    while another thread creates and destroys simulations (device allocations,
+    releases, streams): every call succeeds and each result equals eager
+    launches bit for bit."""
+    import threading
+    announce("graph capture beside create/destroy on another thread")
+    sim, b = make(SCENES[0])
+    sim.set_graph_steps(0)
+    sim.coupled(b, 12)
+    ref = full_state(sim, b)
+    sim.close()
+    stop = threading.Event()
+    churn = [0]
+    errors = []
+
+    def churner():
+        try:
+            while not stop.is_set():
+                s = lbm.Sim(48, 40, 44)
+                s.close()
+                churn[0] += 1
+        except Exception as e:  # noqa: BLE001 (reported below)
+            errors.append(e)
+
+    th = threading.Thread(target=churner)
+    th.start()
+    try:
+        for rep in range(12):
+            sim, b = make(SCENES[0])
+            sim.set_graph_steps(2)
+            sim.coupled(b, 12)
+            caps, reps = sim.graph_counts()
+            assert caps == 1 and reps >= 1, (rep, caps, reps)
+            assert_identical(full_state(sim, b), ref, f"capture beside churn {rep}")
+            sim.close()
+    finally:
+        stop.set()
+        th.join()
+    assert not errors, errors
+    assert churn[0] > 0
+    announce("simulations created and destroyed meanwhile:", churn[0])

@@ -80,6 +80,9 @@ def test_per_pass_absorb_and_pack():
 
 
 def run_golden(name, sim, b, nsteps, fields, fused, sphere_exact=True, field_tol=None):
+    """Steps sim one call per step against the reference's recorded run: the
+    sphere every step, the fresh and dead cell lists every step (when recorded),
+    and the fields at the recorded snapshot steps."""
     meta, A = golden_io.load(name)
     if b is not None:
         assert b.as_list() == meta["sphere0"]
@@ -96,6 +99,11 @@ def run_golden(name, sim, b, nsteps, fields, fused, sphere_exact=True, field_tol
             else:
                 d = max(abs(b.cx - ref[3]), abs(b.cy - ref[4]), abs(b.cz - ref[5]))
                 assert d <= 1e-4, (t, d)
+        if "fresh" in meta:
+            # The reference's fresh and dead cells of this step's retag
+            # (src/sim.js:246-256, src/solid.js:87-101), in visiting order.
+            assert list(sim.fresh_cells()) == meta["fresh"][t - 1], ("fresh", t)
+            assert list(sim.dead_cells()) == meta["dead"][t - 1], ("dead", t)
         if t in meta["snap_steps"]:
             tag = sim.get("tag")
             if "tag" in fields:
