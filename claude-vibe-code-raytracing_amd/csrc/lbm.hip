@@ -161,6 +161,39 @@ static hipError_t host_free(void *p) {
   return hipHostFree(p);
 }
 
+// ------------------------------------------------------------------ no-alloc scratch
+// Device memory of lbm_step_noalloc: the fused step's link-momentum partials
+// (6 doubles per 256-cell tile) in a file-scope device array instead of the
+// handle's own buffer, so that entry point allocates nothing.  Every GPU the
+// library runs on holds its own copy, at its own address, until the process
+// ends.  On the host the array's name is a stand-in, not a device address: host
+// code uses only the address hipGetSymbolAddress gives for each device, looked
+// up once per device by lbm_create (with that device current).
+constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+static bool g_scratch_done[kMaxDevices];
+
+// Looks up the current device's copy of `scratch` once; a failed lookup leaves
+// that device without one (lbm_step_noalloc then reports it).
+static void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_done[device]) return;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+  g_scratch_done[device] = true;
+}
+
+static void *scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
 // ------------------------------------------------------------------ helpers
 
 #define HIP_TRY(sim, expr)                                                              \
@@ -361,6 +394,7 @@ extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
   s->g.n = n;
   s->g.S = (n + 63) / 64 * 64;
   hipGetDevice(&s->device);
+  scratch_setup(s->device);
   // src/sim.js:88-95 defaults.
   s->params = {0.6, 0.7, 0.012, 0.0010, 1.0, 0.05, 4.0, 0.02};
   int cap = 1;

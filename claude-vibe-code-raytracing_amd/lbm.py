@@ -133,6 +133,7 @@ def lib():
         "lbm_sim_step": (C.c_int, [P, SP]),
         "lbm_absorb_dead_cells": (C.c_int, [P, SP]),
         "lbm_step": (C.c_int, [P, SP, C.c_int]),
+        "lbm_step_noalloc": (C.c_int, [P, SP, C.c_int]),
         "lbm_field_count": (C.c_int64, [P, C.c_int]),
         "lbm_download": (C.c_int, [P, C.c_int, P]),
         "lbm_upload": (C.c_int, [P, C.c_int, P]),
@@ -358,6 +359,10 @@ class Sim:
     def coupled(self, b, nsteps):
         """nsteps fused coupled steps (lbm_step)."""
         self._ck(self.L.lbm_step(self.h, self._sp(b), int(nsteps)), "lbm_step")
+
+    def coupled_noalloc(self, b, nsteps):
+        """lbm_step_noalloc: the same steps without device allocation."""
+        self._ck(self.L.lbm_step_noalloc(self.h, self._sp(b), int(nsteps)), "lbm_step_noalloc")
 
     # -- fields
     def get(self, name):

@@ -182,6 +182,18 @@ int lbm_absorb_dead_cells(lbm_sim *sim, lbm_sphere *solid);
  * per-pass path in the last bits.  solid may be NULL (no body). */
 int lbm_step(lbm_sim *sim, lbm_sphere *solid, int nsteps);
 
+/* lbm_step that allocates no device memory.  No reference counterpart: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same kernels and results as
+ * lbm_step, bit for bit; the fused step's link-momentum partials go to a
+ * library-wide device array of 16 MiB held on each GPU until the process ends
+ * (grids of at most 89,477,120 cells, 384^3 included; LBM_ERR_CAPACITY beyond),
+ * and steps are launched one by one (no launch graph).  The handle's
+ * own buffers must already exist (they do after lbm_create, and after
+ * lbm_set_storage for 16-bit storage); timing must be off.  Calls that overlap
+ * on one GPU share that array and race: run them one at a time per GPU. */
+int lbm_step_noalloc(lbm_sim *sim, lbm_sphere *solid, int nsteps);
+
 /* ------------------------------------------------------------------ storage */
 /* Population storage of the fused step.  No reference counterpart (the
  * reference stores float32, src/sim.js:35-38).  LBM_STORAGE_F32 is the
