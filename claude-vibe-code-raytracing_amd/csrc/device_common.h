@@ -52,6 +52,7 @@ struct Ctrl {
 constexpr int kErrListOverflow = 1;
 constexpr int kErrBandEscape = 2;
 constexpr int kErrFreshCapacity = 4;
+constexpr int kErrBandTiles = 8;  // a band-tile launch did not cover every band-row tile
 
 // Tiles of 256 consecutive cells in the x-fastest order (four waves of 64),
 // launched on a 1-D grid in XCD-aware order.

@@ -631,6 +631,7 @@ struct Scene {
   int use_bf;               // external force armed for the first step of the call
   int cgx, cgy;             // full-grid blocks of 64x4 cells: cgx * cgy * nz
   int band_blocks;          // blocks of 256 cells covering the largest band box
+  int band_tile_cols;       // band-row tiles per plane a band-tile launch covers
 };
 
 // Buffer of parity q (0 or 1) of a pair, as codec elements.  Both pointers are
@@ -907,6 +908,33 @@ __global__ __launch_bounds__(256) void k1_phase_stream(const Scene sc, int q) {
 
 // ---------------------------------------------------------------- stream + collide
 
+// Tile of block b of a launch over the band-row tiles (the tiles that hold a
+// cell of the band box's rows, as tile_in_box_rows finds them and the body
+// update reads their momentum): block b takes plane box[4] + b / M and, in it,
+// the (b % M)-th of the plane's band-row tiles not already counted for the
+// plane before (a tile can span two planes).  M = band_tile_cols; the host
+// sizes the launch for the largest box.  A plane with more band-row tiles than
+// M, or a box with more planes than the launch covers, sets kErrBandTiles.
+__device__ __forceinline__ bool band_tile(const Scene &sc, const int box[6], int b, int &tile) {
+  const int M = sc.band_tile_cols;
+  if (box[5] < box[4]) return false;
+  if (b == 0 && (box[5] - box[4] + 1) * M > (int)gridDim.x) atomicOr(&sc.ctrl->err, kErrBandTiles);
+  const int z = box[4] + b / M;
+  if (z > box[5]) return false;
+  const TileMap &tm = sc.tm;
+  const int64_t p = (int64_t)z * tm.nxny;
+  int lo = (int)((p + (int64_t)box[2] * tm.nx + box[0]) >> kTileShift);
+  const int hi = (int)((p + (int64_t)box[3] * tm.nx + box[1]) >> kTileShift);
+  if (z > box[4]) {
+    const int hprev = (int)((p - tm.nxny + (int64_t)box[3] * tm.nx + box[1]) >> kTileShift);
+    lo = lo > hprev + 1 ? lo : hprev + 1;
+  }
+  const int c = b % M;
+  if (c == M - 1 && lo + c < hi) atomicOr(&sc.ctrl->err, kErrBandTiles);
+  tile = lo + c;
+  return tile <= hi;
+}
+
 // stream(t) + macro(t) for every fluid cell of step q; with COLLIDE, also
 // collide(t+1) for fluid cells outside the band.  Without COLLIDE (the last
 // step of a call) the post-stream populations are stored for every cell, which
@@ -1075,15 +1103,36 @@ __device__ __forceinline__ void finish_cell_dev(const Scene &sc, int64_t i, bool
 // stream(t) + macro(t) for every fluid cell, then collide(t+1) outside the
 // band; one cell per lane, tiles of 256 cells.  COLLIDE false: the stream-only
 // step that ends a call (phi(t+1) written here, rho/u stored everywhere).
-template <bool COLLIDE, int KIND>
+// MODE: kTilesAll covers every tile; kTilesBulk every tile but the band-row
+// tiles, whose cells all lie outside the band box (no tag is read, no link
+// momentum arises: the kernel carries neither path); kTilesBand only the
+// band-row tiles (band_tile).  Bulk and band launches together cover every
+// tile once.
+constexpr int kTilesAll = 0, kTilesBulk = 1, kTilesBand = 2;
+template <bool COLLIDE, int KIND, int MODE = kTilesAll>
 __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int q, int first) {
   using C = Codec<KIND>;
   using R = typename C::R;
   using T = typename C::T;
   const Geom &g = sc.g;
   const Derived &d = sc.d;
+  const Ctrl *__restrict__ ctrl = sc.ctrl;
+  int box[6];
+#pragma unroll
+  for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
   int tile, x, y, z;
-  if (!map_tile(sc.tm.T, sc.tm.T8, lb, tile)) return;
+  if constexpr (MODE == kTilesBand) {
+    if (!band_tile(sc, box, lb, tile)) return;
+  } else {
+    if (!map_tile(sc.tm.T, sc.tm.T8, lb, tile)) return;
+  }
+  // Link momentum can only arise next to the sphere, inside the band box: the
+  // tiles holding the box's rows store their sums (zero or not), and the body
+  // update reads exactly those.
+  const bool box_rows = MODE == kTilesBand || (MODE == kTilesAll && tile_in_box_rows(sc.tm, box, tile));
+  if constexpr (MODE == kTilesBulk) {
+    if (tile_in_box_rows(sc.tm, box, tile)) return;
+  }
   const int64_t i = ((int64_t)tile << kTileShift) + threadIdx.y * 64 + threadIdx.x;
   const bool in_grid = i < g.n;
   cell_xyz(g.nx, g.nxny, i, x, y, z);
@@ -1095,15 +1144,8 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
   const float *__restrict__ phi = par(sc.phi, q);
   float *__restrict__ phi_next = par(sc.phi, q ^ 1);
   const uint8_t *__restrict__ tag = sc.tag;
-  const Ctrl *__restrict__ ctrl = sc.ctrl;
   double mom[6] = {0, 0, 0, 0, 0, 0};
-  int box[6];
-#pragma unroll
-  for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
-  // Link momentum can only arise next to the sphere, inside the band box: the
-  // tiles holding the box's rows store their sums (zero or not), and the body
-  // update reads exactly those.
-  const bool exchange = sc.has_solid != 0 && tile_in_box_rows(sc.tm, box, tile);
+  const bool exchange = sc.has_solid != 0 && box_rows;
   const int use_bf = first && sc.use_bf;
   double cx = 0, cy = 0, cz = 0;
   if (exchange) {
@@ -1114,7 +1156,7 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
   if (in_grid) {
     // Solid cells lie inside the band box with a margin of 2, so outside it the
     // only non-fluid cells are the walls on the grid faces.
-    const bool near = in_box(box, x, y, z);
+    const bool near = MODE != kTilesBulk && in_box(box, x, y, z);
     const uint32_t bo4 = (uint32_t)i * 4u;
     const bool fluid = interior(g, x, y, z) && (!near || ldb(tag, (uint32_t)i) == TAG_FLUID);
     if (!fluid) {
@@ -1201,6 +1243,17 @@ template <bool COLLIDE, int KIND>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k1_stream_collide(
     const Scene sc, int q, int first) {
   stream_collide_body<COLLIDE, KIND>(sc, blockIdx.x, q, first);
+}
+// The same over the bulk tiles and over the band-row tiles (one scene).
+template <bool COLLIDE, int KIND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k1_stream_collide_bulk(
+    const Scene sc, int q, int first) {
+  stream_collide_body<COLLIDE, KIND, kTilesBulk>(sc, blockIdx.x, q, first);
+}
+template <bool COLLIDE, int KIND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k1_stream_collide_band(
+    const Scene sc, int q, int first) {
+  stream_collide_body<COLLIDE, KIND, kTilesBand>(sc, blockIdx.x, q, first);
 }
 // ---------------------------------------------------------------- stream + collide, two cells per lane
 

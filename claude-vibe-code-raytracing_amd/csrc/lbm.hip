@@ -130,6 +130,9 @@ struct lbm_sim {
   Timer timer;
   GraphCache graphs;  // steady steps of the fused call (lbm_step)
   int lanes = 2;      // cells per lane of the 16-bit stream kernel (1 or 2)
+  // Float32 fused steps launch the bulk tiles and the band-row tiles of the
+  // stream kernel separately (1) or as one launch (0).
+  int stream_split = 1;
   std::string err;
 };
 
@@ -1123,6 +1126,13 @@ extern "C" int lbm_set_graph_steps(lbm_sim *s, int steps) {
   if (rc) return rc;
   s->graphs.clear();
   s->graphs.steps = steps;
+  return LBM_OK;
+}
+
+extern "C" int lbm_set_stream_split(lbm_sim *s, int on) {
+  CHECK_SIM(s);
+  if (on != 0 && on != 1) return fail(s, LBM_ERR_ARG, "stream split must be 0 or 1");
+  s->stream_split = on;
   return LBM_OK;
 }
 

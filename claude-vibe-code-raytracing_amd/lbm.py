@@ -134,6 +134,7 @@ def lib():
         "lbm_absorb_dead_cells": (C.c_int, [P, SP]),
         "lbm_step": (C.c_int, [P, SP, C.c_int]),
         "lbm_step_noalloc": (C.c_int, [P, SP, C.c_int]),
+        "lbm_set_stream_split": (C.c_int, [P, C.c_int]),
         "lbm_field_count": (C.c_int64, [P, C.c_int]),
         "lbm_download": (C.c_int, [P, C.c_int, P]),
         "lbm_upload": (C.c_int, [P, C.c_int, P]),
@@ -448,6 +449,9 @@ class Sim:
     def set_graph_steps(self, steps):
         """Steps per captured launch graph of coupled(); 0 launches kernels one by one."""
         self._ck(self.L.lbm_set_graph_steps(self.h, int(steps)), "lbm_set_graph_steps")
+
+    def set_stream_split(self, on):
+        self._ck(self.L.lbm_set_stream_split(self.h, int(on)), "lbm_set_stream_split")
 
     def set_stream_lanes(self, cells_per_lane):
         """Cells per lane (1 or 2) of the 16-bit stream kernel."""

@@ -419,6 +419,13 @@ int lbm_graph_counts(const lbm_sim *sim, int64_t *captures, int64_t *replays);
  * unspecified); float32 storage always runs one cell per lane. */
 int lbm_set_stream_lanes(lbm_sim *sim, int cells_per_lane);
 
+/* Launch shape of the float32 fused stream kernel for one simulation: 1
+ * (default) launches the tiles away from the sphere by a kernel that carries
+ * no tag reads and no momentum exchange, and the tiles of the sphere's band
+ * rows by the full kernel; 0 launches every tile with the full kernel.  No
+ * reference counterpart; results are the same bit for bit. */
+int lbm_set_stream_split(lbm_sim *sim, int on);
+
 /* Measured device bandwidth of a plain copy: copies `bytes` (rounded down to 16)
  * between two device buffers `reps` times with one float4 per thread and
  * returns the read+write rate in GB/s in *gbps.  No reference counterpart: it
