@@ -469,11 +469,47 @@ __device__ __forceinline__ T pull_elem(const T *__restrict__ p2, int64_t S, int 
   return ldb(base, bounce ? own : src);
 }
 
+// Float32 populations (row blocks) of the cells of one tile of 256 consecutive
+// cells: a wave-uniform element base (row_elem of the tile's first cell) and
+// each lane's 32-bit byte offset from it to its own cell's direction-0 element,
+// so that every load and store adds one per-lane offset to a uniform base and
+// needs no 64-bit address arithmetic per lane.
+struct RowAddr {
+  int64_t base;
+  uint32_t relb;
+};
+// i0: the tile's first cell; i, r: the lane's cell and its row (z ny + y).
+__device__ __forceinline__ RowAddr row_addr(const Geom &g, int64_t i0, int64_t i, int r) {
+  const int r0 = (int)(i0 / g.nx);
+  RowAddr a;
+  a.base = i0 + (int64_t)r0 * (kRowQ - 1) * g.nx;
+  a.relb = (uint32_t)((i - i0) + (int64_t)(r - r0) * (kRowQ - 1) * g.nx) * 4u;
+  return a;
+}
+// Direction k (opposite kopp, source (x - ex, y - ey, z - ez)) of a cell, from
+// the row-block pointer p (the f pointer, or the h pointer for D3Q7): the
+// source's element, or at a non-fluid source the cell's own element of the
+// opposite direction.  Both candidates are uniform offsets from the lane's own
+// element; the uniform base is the lower of them, and the per-lane choice is a
+// 32-bit byte offset.  Needs the plane span (ny + 2) kRowQ nx of 4-byte elements
+// below 2^31 bytes (checked on the host), and the source inside the grid, which
+// holds for fluid cells.
+__device__ __forceinline__ float pull_row(const float *__restrict__ p, const RowAddr &a,
+                                          const Geom &g, int k, int kopp, int ex, int ey, int ez,
+                                          bool bounce) {
+  const int64_t cs = (int64_t)k * g.nx - row_src_delta(g.nx, g.ny, ex, ey, ez);
+  const int64_t co = (int64_t)kopp * g.nx;
+  const int64_t lo = cs < co ? cs : co;
+  const float *b = p + a.base + lo;
+  const uint32_t ds = (uint32_t)(cs - lo) * 4u, dop = (uint32_t)(co - lo) * 4u;
+  return ldb(b, a.relb + (bounce ? dop : ds));
+}
+
 // Pull stream of the 27 populations with halfway bounce-back, given the source
 // masks.  Every load address is chosen without branching (own opposite
 // population at a non-fluid source), so the 27 loads are in flight together.
 template <int KIND>
-__device__ __forceinline__ void pull27(const Geom &g, int64_t i,
+__device__ __forceinline__ void pull27(const Geom &g, int64_t i, const RowAddr &ra,
                                        const typename Codec<KIND>::T *__restrict__ f2,
                                        uint32_t nonfluid, uint32_t solid,
                                        const float *__restrict__ usx,
@@ -488,10 +524,9 @@ __device__ __forceinline__ void pull27(const Geom &g, int64_t i,
   static_assert(KIND == ST_F32, "16-bit codecs pull in deviation form");
   using C = Codec<KIND>;
   float raw[Q27];
-  const uint32_t bo = (uint32_t)i * 4u;
 #pragma unroll
   for (int k = 0; k < Q27; k++)
-    raw[k] = pull_elem(f2, g.S, k, (int)dir_off(g, k), bo, (nonfluid >> k) & 1u);
+    raw[k] = pull_row(f2, ra, g, k, opp27(k), EX27[k], EY27[k], EZ27[k], (nonfluid >> k) & 1u);
   links27<typename C::R>(g, i, raw, solid, usx, usy, usz, rhoPhaseOld, exchange, x, y, z, cx, cy,
                          cz, mom, fp, pSumF, mx, my, mz);
 }
@@ -562,14 +597,13 @@ __device__ __forceinline__ void links27_dev(const Geom &g, int64_t i, Get &&get,
 // (D3Q7 directions 1..6 are D3Q27 directions 1..6, so the D3Q27 source mask
 // serves).
 template <int KIND>
-__device__ __forceinline__ void pull7(const Geom &g, int64_t i,
+__device__ __forceinline__ void pull7(const Geom &g, const RowAddr &ra,
                                       const typename Codec<KIND>::T *__restrict__ h2,
                                       uint32_t nonfluid, float hp[Q7]) {
   static_assert(KIND == ST_F32, "16-bit codecs pull in deviation form");
-  const uint32_t bo = (uint32_t)i * 4u;
 #pragma unroll
   for (int k = 0; k < Q7; k++)
-    hp[k] = pull_elem(h2, g.S, k, (int)dir_off(g, k), bo, (nonfluid >> k) & 1u);
+    hp[k] = pull_row(h2, ra, g, k, opp7(k), EX7[k], EY7[k], EZ7[k], (nonfluid >> k) & 1u);
 }
 
 // Sum of a double over the 64 lanes of a wave by a fixed butterfly; every lane
@@ -796,15 +830,17 @@ __device__ __forceinline__ void collide_body(const Scene &sc, int lb, int p) {
                       [&](int k, float v) { C::std_(fo + (int64_t)k * g.S, bo, v); });
     return;
   }
+  // Float32: row blocks.
+  const int64_t pe = row_elem(g, i);
   float hin[Q7], hc[Q7];
 #pragma unroll
-  for (int k = 0; k < Q7; k++) hin[k] = C::ldo(hi + (int64_t)k * g.S, bo, w7(k) * phc);
+  for (int k = 0; k < Q7; k++) hin[k] = C::ldo(hi + pe + (int64_t)k * g.nx, 0u, w7(k) * phc);
   collide_phase_cell<R>(sc.d, pc, uxe, uye, uze, (R)gx1, (R)gy1, (R)gz1, hin, hc);
 #pragma unroll
-  for (int k = 0; k < Q7; k++) C::sto(ho + (int64_t)k * g.S, bo, hc[k], w7(k) * phc);
+  for (int k = 0; k < Q7; k++) C::sto(ho + pe + (int64_t)k * g.nx, 0u, hc[k], w7(k) * phc);
   float fin[Q27], fc[Q27];
 #pragma unroll
-  for (int k = 0; k < Q27; k++) fin[k] = C::ldo(fi + (int64_t)k * g.S, bo, w27(k));
+  for (int k = 0; k < Q27; k++) fin[k] = C::ldo(fi + pe + (int64_t)k * g.nx, 0u, w27(k));
   const bool bf = !BAND && sc.use_bf;
   const R bx = bf ? (R)sc.bfx[i] : R(0);
   const R by = bf ? (R)sc.bfy[i] : R(0);
@@ -812,7 +848,7 @@ __device__ __forceinline__ void collide_body(const Scene &sc, int lb, int p) {
   collide_hydro_cell<R>(sc.d, (R)sc.rho[i], pc, (R)lap1, (R)gx1, (R)gy1, (R)gz1, bx, by, bz, uxe,
                         uye, uze, fin, fc);
 #pragma unroll
-  for (int k = 0; k < Q27; k++) C::sto(fo + (int64_t)k * g.S, bo, fc[k], w27(k));
+  for (int k = 0; k < Q27; k++) C::sto(fo + pe + (int64_t)k * g.nx, 0u, fc[k], w27(k));
 }
 template <bool BAND, int KIND>
 __global__ __launch_bounds__(256) void km_collide(const Scene *__restrict__ scenes,
@@ -886,7 +922,7 @@ __device__ __forceinline__ void phase_stream_body(const Scene &sc, int lb, int q
     stb(phi_next, bo4, phase_sum_dev(e, phib, hfull));
   } else {
     float hv[Q7];
-    pull7<KIND>(g, i, h2, nonfluid, hv);
+    pull7<KIND>(g, row_addr(g, (int64_t)tile << kTileShift, i, z * g.ny + y), h2, nonfluid, hv);
     R phiSum = 0;
 #pragma unroll
     for (int k = 0; k < Q7; k++) phiSum += (R)hv[k];
@@ -1217,17 +1253,18 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
         nonfluid = wall_mask(g, x, y, z);
         solid = 0;
       }
+      const RowAddr ra = row_addr(g, (int64_t)tile << kTileShift, i, z * g.ny + y);
       float fp[Q27];
       R pSumF, mx, my, mz;
-      pull27<KIND>(g, i, f2, nonfluid, solid, sc.usx, sc.usy, sc.usz, rhoPhaseOld, exchange, x, y,
-                   z, cx, cy, cz, mom, fp, pSumF, mx, my, mz);
+      pull27<KIND>(g, i, ra, f2, nonfluid, solid, sc.usx, sc.usy, sc.usz, rhoPhaseOld, exchange,
+                   x, y, z, cx, cy, cz, mom, fp, pSumF, mx, my, mz);
       float hp[Q7];
-      pull7<KIND>(g, i, h2, nonfluid, hp);
-      const uint32_t bo = (uint32_t)i * (uint32_t)sizeof(T);
+      pull7<KIND>(g, ra, h2, nonfluid, hp);
+      T *fob = fo + ra.base, *hob = ho + ra.base;
       finish_cell<COLLIDE, KIND>(
           sc, i, near, phi_next, gxo, gyo, gzo, lapo, fp, hp, pSumF, mx, my, mz, use_bf, phi_next,
-          [&](int k, float val, float base) { C::sto(fo + (int64_t)k * g.S, bo, val, base); },
-          [&](int k, float val, float base) { C::sto(ho + (int64_t)k * g.S, bo, val, base); });
+          [&](int k, float val, float base) { C::sto(fob + (int64_t)k * g.nx, ra.relb, val, base); },
+          [&](int k, float val, float base) { C::sto(hob + (int64_t)k * g.nx, ra.relb, val, base); });
     }
   }
   if (exchange) tile_momentum(mom, sc.partials, tile);
@@ -1689,35 +1726,38 @@ __global__ __launch_bounds__(128) void k1_phase_stream2(const Scene sc, int q) {
 
 // Float32 populations (the reference's f and h) to a 16-bit codec, and back.
 // The h baseline is phi of the cell (the phi the populations belong to).
+// Float32 populations are row blocks, the 16-bit ones planes of stride S.
 template <int KIND>
-__global__ __launch_bounds__(256) void k_encode(int64_t n, int64_t S, const float *__restrict__ f,
+__global__ __launch_bounds__(256) void k_encode(Geom g, const float *__restrict__ f,
                                                 const float *__restrict__ h,
                                                 const float *__restrict__ phi,
                                                 typename Codec<KIND>::T *__restrict__ cf,
                                                 typename Codec<KIND>::T *__restrict__ ch) {
   using C = Codec<KIND>;
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+  if (i >= g.n) return;
   const float pc = phi[i];
+  const int64_t pe = row_elem(g, i), S = g.S;
 #pragma unroll
-  for (int k = 0; k < Q27; k++) C::st(cf, k * S + i, f[k * S + i], w27(k));
+  for (int k = 0; k < Q27; k++) C::st(cf, k * S + i, f[pe + (int64_t)k * g.nx], w27(k));
 #pragma unroll
-  for (int k = 0; k < Q7; k++) C::st(ch, k * S + i, h[k * S + i], w7(k) * pc);
+  for (int k = 0; k < Q7; k++) C::st(ch, k * S + i, h[pe + (int64_t)k * g.nx], w7(k) * pc);
 }
 template <int KIND>
-__global__ __launch_bounds__(256) void k_decode(int64_t n, int64_t S,
+__global__ __launch_bounds__(256) void k_decode(Geom g,
                                                 const typename Codec<KIND>::T *__restrict__ cf,
                                                 const typename Codec<KIND>::T *__restrict__ ch,
                                                 const float *__restrict__ phi,
                                                 float *__restrict__ f, float *__restrict__ h) {
   using C = Codec<KIND>;
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+  if (i >= g.n) return;
   const float pc = phi[i];
+  const int64_t pe = row_elem(g, i), S = g.S;
 #pragma unroll
-  for (int k = 0; k < Q27; k++) f[k * S + i] = C::ld(cf, k * S + i, w27(k));
+  for (int k = 0; k < Q27; k++) f[pe + (int64_t)k * g.nx] = C::ld(cf, k * S + i, w27(k));
 #pragma unroll
-  for (int k = 0; k < Q7; k++) h[k * S + i] = C::ld(ch, k * S + i, w7(k) * pc);
+  for (int k = 0; k < Q7; k++) h[pe + (int64_t)k * g.nx] = C::ld(ch, k * S + i, w7(k) * pc);
 }
 
 // Global mass check: sums over interior fluid cells of phi and of rho (the

@@ -76,7 +76,9 @@ struct lbm_sim {
   int device = 0;
   hipStream_t stream = nullptr;
   lbm_params params{};
-  // Population buffers: fbuf[fcur] is the reference's f, fbuf[1 - fcur] its f2;
+  // Population buffers (row blocks of the 27 f and 7 h directions, one per step
+  // parity; hbuf[b] points into fbuf[b]): fbuf[fcur] is the reference's f,
+  // fbuf[1 - fcur] its f2;
   // likewise for h.  phibuf[phicur] is phi, phibuf[1 - phicur] phiPrev.
   float *fbuf[2] = {nullptr, nullptr};
   float *hbuf[2] = {nullptr, nullptr};
@@ -264,13 +266,12 @@ static int ensure_scratch(lbm_sim *s, size_t bytes) {
 // phi) into fbuf[fcur] and hbuf[hcur], on stream st.
 static int ensure_f32(lbm_sim *s, hipStream_t st) {
   if (!s->comp_auth) return LBM_OK;
-  const int64_t n = s->g.n, S = s->g.S;
-  const unsigned blocks = (unsigned)((n + 255) / 256);
+  const unsigned blocks = (unsigned)((s->g.n + 255) / 256);
   if (s->storage == LBM_STORAGE_F16)
-    k_decode<ST_F16><<<blocks, 256, 0, st>>>(n, S, (const _Float16 *)s->cf[s->ccur],
+    k_decode<ST_F16><<<blocks, 256, 0, st>>>(s->g, (const _Float16 *)s->cf[s->ccur],
                                              (const _Float16 *)s->ch[s->ccur], PHI(s), F(s), H(s));
   else
-    k_decode<ST_BF16><<<blocks, 256, 0, st>>>(n, S, (const uint16_t *)s->cf[s->ccur],
+    k_decode<ST_BF16><<<blocks, 256, 0, st>>>(s->g, (const uint16_t *)s->cf[s->ccur],
                                               (const uint16_t *)s->ch[s->ccur], PHI(s), F(s),
                                               H(s));
   hipError_t e = hipGetLastError();
@@ -409,10 +410,8 @@ extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
   if (e == hipSuccess) {                                         \
     e = dev_malloc((void **)&(ptr), (bytes));                     \
   }
-  ALLOC(s->fbuf[0], S * 27 * sizeof(float));
-  ALLOC(s->fbuf[1], S * 27 * sizeof(float));
-  ALLOC(s->hbuf[0], S * 7 * sizeof(float));
-  ALLOC(s->hbuf[1], S * 7 * sizeof(float));
+  ALLOC(s->fbuf[0], n * kRowQ * sizeof(float));
+  ALLOC(s->fbuf[1], n * kRowQ * sizeof(float));
   ALLOC(s->phibuf[0], S * sizeof(float));
   ALLOC(s->phibuf[1], S * sizeof(float));
   ALLOC(s->rho, S * sizeof(float));
@@ -436,14 +435,15 @@ extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
   ALLOC(s->level_scratch, (size_t)cap * sizeof(int));
   ALLOC(s->ctrl, sizeof(Ctrl));
 #undef ALLOC
+  if (e == hipSuccess)
+    for (int b = 0; b < 2; b++) s->hbuf[b] = s->fbuf[b] + (int64_t)Q27 * nx;
   if (e == hipSuccess) e = host_malloc((void **)&s->ctrl_host, sizeof(Ctrl), hipHostMallocDefault);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e == hipSuccess) {
     // Zero the whole state so reads before an init are defined.
     const int64_t fb = S * sizeof(float);
     for (int b = 0; b < 2; b++) {
-      hipMemsetAsync(s->fbuf[b], 0, fb * 27, s->stream);
-      hipMemsetAsync(s->hbuf[b], 0, fb * 7, s->stream);
+      hipMemsetAsync(s->fbuf[b], 0, n * kRowQ * sizeof(float), s->stream);
       hipMemsetAsync(s->phibuf[b], 0, fb, s->stream);
     }
     float *arrs[] = {s->rho, s->ux, s->uy, s->uz, s->usx, s->usy, s->usz, s->gx, s->gy, s->gz,
@@ -468,7 +468,7 @@ extern "C" void lbm_destroy(lbm_sim *s) {
   std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
   if (s->stream) hipStreamSynchronize(s->stream);
   collect_timing(s->timer);
-  void *ptrs[] = {s->fbuf[0], s->fbuf[1], s->hbuf[0], s->hbuf[1], s->phibuf[0], s->phibuf[1],
+  void *ptrs[] = {s->fbuf[0], s->fbuf[1], s->phibuf[0], s->phibuf[1],
                   s->rho, s->ux, s->uy, s->uz, s->tag, s->tagPrev, s->usx, s->usy, s->usz, s->gx,
                   s->gy, s->gz, s->lap, s->bfx, s->bfy, s->bfz, s->fresh, s->dead,
                   s->level_scratch, s->ctrl, s->link_key, s->link_m, s->imp, s->scratch,
@@ -885,9 +885,9 @@ extern "C" int lbm_download(lbm_sim *s, int field, void *host) {
   } else {
     int rc = ensure_scratch(s, n * q * sizeof(float));
     if (rc) return rc;
-    k_soa_to_aos<<<(n + 255) / 256, 256, 0, s->stream>>>(n, s->g.S, q, (const float *)dp,
-                                                         (float *)s->scratch);
-    if ((rc = check_launch(s, "k_soa_to_aos"))) return rc;
+    k_rows_to_aos<<<(n + 255) / 256, 256, 0, s->stream>>>(s->g, q, (const float *)dp,
+                                                          (float *)s->scratch);
+    if ((rc = check_launch(s, "k_rows_to_aos"))) return rc;
     HIP_TRY(s, hipMemcpyAsync(host, s->scratch, n * q * sizeof(float), hipMemcpyDeviceToHost,
                               s->stream));
   }
@@ -909,9 +909,9 @@ extern "C" int lbm_upload(lbm_sim *s, int field, const void *host) {
     if ((rc = ensure_scratch(s, n * q * sizeof(float)))) return rc;
     HIP_TRY(s, hipMemcpyAsync(s->scratch, host, n * q * sizeof(float), hipMemcpyHostToDevice,
                               s->stream));
-    k_aos_to_soa<<<(n + 255) / 256, 256, 0, s->stream>>>(n, s->g.S, q, (const float *)s->scratch,
-                                                         (float *)dp);
-    if ((rc = check_launch(s, "k_aos_to_soa"))) return rc;
+    k_aos_to_rows<<<(n + 255) / 256, 256, 0, s->stream>>>(s->g, q, (const float *)s->scratch,
+                                                          (float *)dp);
+    if ((rc = check_launch(s, "k_aos_to_rows"))) return rc;
   }
   if (field == LBM_FIELD_BODYFX || field == LBM_FIELD_BODYFY || field == LBM_FIELD_BODYFZ)
     s->bodyF_armed = true;
@@ -1159,14 +1159,14 @@ __global__ __launch_bounds__(256) void k_copy_probe(const float4 *__restrict__ a
 }
 
 // The population pattern of the fused stream kernels with no arithmetic: every
-// one of the 27 + 7 planes pulled from its shifted source (the own element
+// one of the 27 + 7 directions pulled from its shifted source (the own element
 // where the source is outside the grid) and stored, non-temporally, in the
-// kernels' tiles and order.  One float32 cell per lane, or two 16-bit cells per
-// lane in aligned 4-byte pairs (an odd offset straddles two pairs).  As in the
-// kernels, every pull is issued before the first store: buffer loads are not
-// moved past buffer stores by the compiler, and a probe that stores each
-// direction after its load runs the 16-bit pattern at 4.2 instead of 5.0 TB/s
-// (256^3, MI355X).
+// kernels' tiles, order and layout: row blocks of 4-byte populations with one
+// cell per lane, or planes of 16-bit populations with two cells per lane in
+// aligned 4-byte pairs (an odd offset straddles two pairs).  As in the kernels,
+// every pull is issued before the first store: buffer loads are not moved past
+// buffer stores by the compiler, and a probe that stores each direction after
+// its load runs the 16-bit pattern at 4.2 instead of 5.0 TB/s (256^3, MI355X).
 __global__ __launch_bounds__(256) void k_pull_probe32(const float *__restrict__ src,
                                                       float *__restrict__ dst, Geom g,
                                                       TileMap tm) {
@@ -1174,17 +1174,23 @@ __global__ __launch_bounds__(256) void k_pull_probe32(const float *__restrict__ 
   if (!map_tile(tm.T, tm.T8, blockIdx.x, tile)) return;
   const int64_t i = ((int64_t)tile << kTileShift) + threadIdx.x;
   if (i >= g.n) return;
-  const uint32_t bo = (uint32_t)i * 4u;
+  int x, y, z;
+  cell_xyz(g.nx, g.nxny, i, x, y, z);
+  const RowAddr ra = row_addr(g, (int64_t)tile << kTileShift, i, z * g.ny + y);
   float v[Q27 + Q7];
 #pragma unroll
   for (int k = 0; k < Q27 + Q7; k++) {
     const int kk = k < Q27 ? k : k - Q27;
-    const int64_t off = (int64_t)EZ27[kk] * g.nxny + (int64_t)EY27[kk] * g.nx + EX27[kk];
-    const bool in = i >= off && i - off < g.n;
-    v[k] = ldb(src + (int64_t)k * g.S, in ? bo - (uint32_t)off * 4u : bo);
+    const int ex = EX27[kk], ey = EY27[kk], ez = EZ27[kk];
+    const bool in = x - ex >= 0 && x - ex < g.nx && y - ey >= 0 && y - ey < g.ny &&
+                    z - ez >= 0 && z - ez < g.nz;
+    // Direction k of the merged row block; the opposite of an h direction is
+    // another h direction.
+    const int kopp = k < Q27 ? opp27(k) : Q27 + opp7(kk);
+    v[k] = pull_row(src, ra, g, k, kopp, ex, ey, ez, !in);
   }
 #pragma unroll
-  for (int k = 0; k < Q27 + Q7; k++) stn(dst + (int64_t)k * g.S, bo, v[k]);
+  for (int k = 0; k < Q27 + Q7; k++) stn(dst + ra.base + (int64_t)k * g.nx, ra.relb, v[k]);
 }
 __global__ __launch_bounds__(128) void k_pull_probe16(const uint16_t *__restrict__ src,
                                                       uint16_t *__restrict__ dst, Geom g,
@@ -1234,7 +1240,7 @@ extern "C" int lbm_probe_pull_bandwidth(int nx, int ny, int nz, int storage, int
   g.S = (g.n + 63) / 64 * 64;
   const int elem = storage == LBM_STORAGE_F32 ? 4 : 2;
   const int64_t bytes = (int64_t)(Q27 + Q7) * g.S * elem;
-  if (elem == 4 ? g.S >= kF32FusedMaxCells : bytes >= ((int64_t)1 << 32)) return LBM_ERR_ARG;
+  if (elem == 4 ? !f32_fused_fits(nx, ny) : bytes >= ((int64_t)1 << 32)) return LBM_ERR_ARG;
   const TileMap tm = make_tile_map(nx, ny, nz);
   void *a = nullptr, *b = nullptr;
   hipStream_t st = nullptr;
