@@ -1,25 +1,32 @@
 """Benchmark of the coupled LBM step on one MI355X.
 
-Workload: 256^3 single scene (liquid 40%, perturbed interface, falling sphere
-r=24), float32 populations, FP64 arithmetic, one coupled step = sim.step;
-absorbDeadCells; sphere step.  Prints one JSON line.
+Workload: the 256^3 single scene of config 1 (liquid 40%, perturbed interface,
+falling sphere r=24), float32 populations, FP64 arithmetic; one coupled step
+= sim.step; absorbDeadCells; sphere step.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps 1000] [--warmup 50] [--size 256]
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--size 256]
 
-MLUPS counts all nx*ny*nz cells, walls included, over a call launched as in
-production (launch shapes from the autotuner, no timing events).  The roofline
-object is for the dominant kernel (stream + macro + next collide), timed with
-HIP events on the library's stream over a second call of the same length right
-after the timed one; its algorithmic traffic is 281 B
-per cell in float32 (27+7 populations read and written, phi(t), phi(t+1) and
-the tag read; bulk cells store rho and u only at the last step of a call) and
-145 B with 16-bit populations (--storage f16/bf16).  The whole-step fraction
-uses 297 B (161 B) per cell (the same plus the phi write of the phase pass and
-the rho/u writes).  `traffic` is the measured HBM bytes per launch from the
-committed counter profiles (profiles/stream_collide_256cube*_traffic.json,
-rocprofv3 FETCH_SIZE/WRITE_SIZE calibrated on a copy kernel), for the 256^3
-scene only.  Other workloads: --workload batch (config 2) and --workload serve
-(config 4, Poisson arrivals into the serving loop).
+MLUPS counts all nx*ny*nz cells, walls included, over the timed steps of a
+continuing run (lbm_step_live: no per-call prologue or closing stream-only
+step inside the timed region, every step complete; `reference_state_call`
+gives the same steps as one call from and to the reference's state).  Launch
+shapes come from the autotuner; no timing events in the timed region.  The
+roofline object is for the dominant kernel (stream + macro + next collide, the
+bulk and band-row launches of a step together), timed with HIP events on the
+library's stream over a further call; its algorithmic traffic is 281 B per
+cell in float32 (27+7 populations read and written, phi(t), phi(t+1) and the
+tag read; bulk cells store rho and u only at the last step of a call) and
+145 B with 16-bit populations.  The whole-step fraction uses 297 B (161 B) per
+cell (the same plus the phi write of the phase pass and the rho/u writes).
+`traffic` is the measured HBM bytes per launch from the committed counter
+profiles (rocprofv3 FETCH_SIZE/WRITE_SIZE calibrated on a copy kernel), for
+the 256^3 scene only.
+
+The default line also carries the FP16-deviation form on the same scene
+(`f16`, with its ratio to float32) and the serving loop (config 4: Poisson
+arrivals into the request loop) in both forms for --serve-seconds each
+(`serve`: served MLUPS, frame latency and time-to-first-frame percentiles).
+Other workloads: --workload batch (config 2) and --workload serve.
 The CPU baseline is the oracle (a single-threaded C restatement of the
 reference) on the same scene for a few steps."""
 import argparse
@@ -205,12 +212,12 @@ def serve_consume(srv, args, t0, lat, first):
     return srv.stats(), time.perf_counter() - t0
 
 
-def bench_serve(args):
+def serve_run(args, storage, duration):
     """Served scenes (config 4): Poisson arrivals, admission by memory budget,
-    continuous batching, an RG8 frame every 10 steps; aggregate MLUPS and frame
-    latency percentiles over the run."""
+    continuous batching, an RG8 frame every 10 steps, for `duration` seconds;
+    returns aggregate MLUPS, frame latency and time-to-first-frame percentiles."""
     import threading
-    reqs = scenegen.config4(args.duration, rate=args.rate, seed=args.seed + 3)
+    reqs = scenegen.config4(duration, rate=args.rate, seed=args.seed + 3)
     srv = lbm.Server(int(args.budget_gb * (1 << 30)), max_active=args.max_active)
     t0 = time.perf_counter()
     stop = threading.Event()
@@ -225,7 +232,7 @@ def bench_serve(args):
                                 sph["density"])
             try:
                 srv.submit(nx, ny, nz, params, steps, frame_every=10, waterline=yw, sphere=b,
-                           storage=args.storage)
+                           storage=storage)
                 submitted[0] += 1
             except lbm.LbmError:
                 pass
@@ -233,43 +240,130 @@ def bench_serve(args):
     th = threading.Thread(target=feeder, daemon=True)
     th.start()
     lat, first = [], []
+    sargs = argparse.Namespace(duration=duration)
     try:
-        st, dt = serve_consume(srv, args, t0, lat, first)
+        st, dt = serve_consume(srv, sargs, t0, lat, first)
     finally:
         stop.set()
         th.join()
         srv.close()
-    frames = len(lat) + len(first)
     mlups = st["lattice_updates"] / dt / 1e6
     pct = lambda v, p: round(float(np.percentile(v, p)) * 1e3, 2) if v else None
-    out = {
-        "metric": "served scenes aggregate MLUPS, p99 frame latency (config 4)",
-        "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1, "steps": None,
-        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": ARITH[args.storage], "data": "synthetic",
-        "config": {"workload": "config 4: Poisson arrivals, axes log-uniform 32..192, step budgets "
-                               "500..5000, RG8 frame every 10 steps, admission by memory budget, "
-                               "continuous batching",
-                   "rate_per_s": args.rate, "duration_s": args.duration,
-                   "budget_gib": args.budget_gb, "max_active": args.max_active,
-                   "storage": args.storage, "seed": args.seed + 3},
+    return {
+        "storage": storage, "mlups": round(mlups, 2), "duration_s": round(dt, 1),
         "frame_latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "count": len(lat)},
         "first_frame_latency_ms": {"p50": pct(first, 50), "p99": pct(first, 99),
                                    "count": len(first)},
         "requests": {"offered": len(reqs), "submitted": submitted[0], "admitted": st["admitted"],
                      "completed": st["completed"], "max_active": st["max_active_seen"]},
         "gpu_busy_frac": round(st["busy_seconds"] / dt, 4),
-        "frames": frames,
+        "frames": len(lat) + len(first),
+    }
+
+
+SERVE_WORKLOAD = ("config 4: Poisson arrivals at 20/s, axes log-uniform 32..192, step budgets "
+                  "500..5000, RG8 frame every 10 steps, admission by memory budget, continuous "
+                  "batching")
+
+
+def bench_serve(args):
+    """The serving workload alone (--workload serve): one JSON line."""
+    r = serve_run(args, args.storage, args.duration)
+    out = {
+        "metric": "served scenes aggregate MLUPS, p99 frame latency (config 4)",
+        "value": r["mlups"], "unit": "MLUPS", "n_gpus": 1, "steps": None,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": ARITH[args.storage], "data": "synthetic",
+        "config": {"workload": SERVE_WORKLOAD, "rate_per_s": args.rate,
+                   "duration_s": args.duration, "budget_gib": args.budget_gb,
+                   "max_active": args.max_active, "storage": args.storage,
+                   "seed": args.seed + 3},
+        "serve": r,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def single_leg(args, storage, copy_gbps):
+    """The config-1 scene on one GPU with populations stored as `storage`:
+    warm-up, the launch-shape autotuner, then the timed steps.
+
+    The timed steps continue a live run (lbm_step_live): the warm-up ends live,
+    the timed call continues it and ends live, so neither the per-call prologue
+    (retag, fresh pass and collide of every cell) nor the stream-only step that
+    puts a call's end state in the reference's form falls inside the timed
+    region; every timed step does the whole coupled step.  The same steps in one
+    call that starts and ends in the reference's state are timed as well
+    (`reference_state_call`).  Kernel times come from a further call with HIP
+    events around every launch on the library's stream."""
+    n = args.size
+    pattern_gbps = lbm.pull_bandwidth(n, n, n, storage, 10)
+    sim, b, params, yw, sph = build_scene(n, args.seed)
+    sim.set_storage(storage)
+    print(f"[bench] {storage}: scene {n}^3 built; autotune, warmup {args.warmup}",
+          file=sys.stderr, flush=True)
+    # Launch shapes from the autotuner (its trial steps are ordinary steps).
+    tune = sim.autotune(b, args.tune_steps) if args.tune_steps > 0 else None
+    # The warm-up is long enough to capture the chosen launch graph.
+    sim.coupled_live(b, max(args.warmup, (tune["graph_steps"] + 3) if tune else 1))
+    print(f"[bench] {storage}: timing {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    sim.coupled_live(b, args.steps)
+    dt = time.perf_counter() - t0
+    # Kernel durations: the next steps with events around every launch (one by
+    # one; the call starts from the live state and ends in the reference's).
+    sim.set_timing(True)
+    t1 = time.perf_counter()
+    sim.coupled(b, args.steps)
+    dt_ev = time.perf_counter() - t1
+    kt = {k: sim.kernel_time(k) for k in ("stream_collide", "phase_stream", "stream_macro",
+                                          "body", "band", "collide_all")}
+    sim.set_timing(False)
+    # One call of the same length from and to the reference's state.
+    t2 = time.perf_counter()
+    sim.coupled(b, args.steps)
+    dt_ref = time.perf_counter() - t2
+    cells = n ** 3
+    mlups = cells * args.steps / dt / 1e6
+    ms_sc, cnt_sc = kt["stream_collide"]
+    avg_s = ms_sc / max(cnt_sc, 1) / 1e3
+    kb = KERNEL_BYTES_PER_LU[storage]
+    achieved = kb * cells / avg_s / 1e9 if cnt_sc else None
+    phi = sim.get("phi")
+    finite = bool(np.isfinite(phi).all())
+    sim.close()
+    return {
+        "storage": storage, "mlups": round(mlups, 2), "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": measured_traffic(n, storage),
+                     "kernel": STREAM_KERNEL[storage], "bytes_per_lu": kb,
+                     "avg_ms": round(avg_s * 1e3, 4),
+                     "timed_over": f"{args.steps} steps with HIP events after the timed call "
+                                   f"({dt_ev / args.steps * 1e3:.4f} ms/step with events)"},
+        "whole_step_hbm_frac": round(STEP_BYTES_PER_LU[storage] * mlups * 1e6 /
+                                     (HBM_PEAK_GBS * 1e9), 4),
+        "pattern_probe": {"gbps": round(pattern_gbps, 1),
+                          "bytes_per_lu": PATTERN_BYTES_PER_LU[storage],
+                          "kernel_frac": round(achieved / pattern_gbps, 4) if achieved else None},
+        "reference_state_call": {"ms_per_step": round(dt_ref / args.steps * 1e3, 4),
+                                 "mlups": round(cells * args.steps / dt_ref / 1e6, 2),
+                                 "note": f"one lbm_step call of {args.steps} steps, with the "
+                                         "per-call prologue and closing stream-only step"},
+        "kernel_ms": {k: {"total": round(v[0], 3), "launches": v[1]} for k, v in kt.items()},
+        "finite": finite,
+        "sphere": {"cx": b.cx, "cy": b.cy, "cz": b.cz, "vx": b.vx, "vy": b.vy, "vz": b.vz},
+        "autotune": tune,
+        "sph_r": sph["r"], "params": params,
+    }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -290,6 +384,11 @@ def main():
     ap.add_argument("--tune-steps", type=int, default=10,
                     help="single and batch: autotuner trial steps per candidate (0: no "
                          "autotuning)")
+    ap.add_argument("--legs", default="f16,serve",
+                    help="single workload: extra legs in the line (f16: the FP16-deviation form "
+                         "on the same scene; serve: the serving loop in both forms)")
+    ap.add_argument("--serve-seconds", type=float, default=30.0,
+                    help="single workload: seconds of each serving leg")
     ap.add_argument("--storage", choices=["f32", "f16", "bf16"], default="f32",
                     help="population storage: f32 (reference-exact, FP64 arithmetic) or a "
                          "16-bit deviation (FP32 arithmetic)")
@@ -307,50 +406,15 @@ def main():
     if args.workload == "serve":
         return bench_serve(args)
     n = args.size
-    pattern_gbps = lbm.pull_bandwidth(n, n, n, args.storage, 10)
-    sim, b, params, yw, sph = build_scene(n, args.seed)
-    sim.set_storage(args.storage)
-    print(f"[bench] scene {n}^3 built; warmup {args.warmup}", file=sys.stderr, flush=True)
-    sim.coupled(b, args.warmup)
-    # Launch shapes from the autotuner (its trial steps are ordinary steps).
-    tune = sim.autotune(b, args.tune_steps) if args.tune_steps > 0 else None
-    sim.synchronize()
-    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
-    # The timed call launches as in production (no timing events; steady steps
-    # replayed from captured graphs when the autotuner chose them).
-    t0 = time.perf_counter()
-    sim.coupled(b, args.steps)
-    sim.synchronize()
-    dt = time.perf_counter() - t0
-    # Per-kernel times from a call of the same length right after, with HIP
-    # events around every launch on the library's stream (launched one by one;
-    # the events add ~2% to that call's step time).
-    print(f"[bench] kernel times over {args.steps} more steps", file=sys.stderr, flush=True)
-    sim.set_timing(True)
-    t1 = time.perf_counter()
-    sim.coupled(b, args.steps)
-    sim.synchronize()
-    dt_ev = time.perf_counter() - t1
-    kt = {k: sim.kernel_time(k) for k in ("stream_collide", "phase_stream", "stream_macro",
-                                          "body", "band", "collide_all")}
-    sim.set_timing(False)
-    cells = n ** 3
-    mlups = cells * args.steps / dt / 1e6
-    ms_sc, cnt_sc = kt["stream_collide"]
-    avg_s = ms_sc / max(cnt_sc, 1) / 1e3
-    kb = KERNEL_BYTES_PER_LU[args.storage]
-    achieved = kb * cells / avg_s / 1e9 if cnt_sc else None
-    # sanity: the state stays finite
-    phi = sim.get("phi")
-    finite = bool((phi == phi).all())
+    head = single_leg(args, args.storage, copy_gbps)
     out = {
         "metric": METRIC,
-        "value": round(mlups, 2),
+        "value": head["mlups"],
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -360,29 +424,39 @@ def main():
                                f"phase + sphere), {STORAGE_NAME[args.storage]} populations",
                    "nx": n, "ny": n, "nz": n, "storage": args.storage,
                    "arithmetic": ARITH[args.storage], "seed": args.seed,
-                   "sphere_r": sph["r"], "params": params},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": measured_traffic(n, args.storage),
-                     "kernel": STREAM_KERNEL[args.storage], "bytes_per_lu": kb,
-                     "avg_ms": round(avg_s * 1e3, 4),
-                     "timed_over": f"{args.steps} steps with HIP events right after the timed "
-                                   f"call ({dt_ev / args.steps * 1e3:.4f} ms/step with events)"},
-        "whole_step_hbm_frac": round(STEP_BYTES_PER_LU[args.storage] * mlups * 1e6 /
-                                     (HBM_PEAK_GBS * 1e9), 4),
+                   "sphere_r": head["sph_r"], "params": head["params"],
+                   "timed": "steps of a continuing run (lbm_step_live), inputs resident in HBM"},
+        "roofline": head["roofline"],
+        "whole_step_hbm_frac": head["whole_step_hbm_frac"],
         "copy_probe_gbps": round(copy_gbps, 1),
-        # The stream kernel's population pattern alone (34 shifted planes pulled
-        # and stored, no arithmetic): the ceiling its memory pattern allows.
-        "pattern_probe": {"gbps": round(pattern_gbps, 1),
-                          "bytes_per_lu": PATTERN_BYTES_PER_LU[args.storage],
-                          "kernel_frac": round(achieved / pattern_gbps, 4) if achieved else None},
-        "kernel_ms": {k: {"total": round(v[0], 3), "launches": v[1]} for k, v in kt.items()},
-        "finite": finite,
-        "sphere": {"cx": b.cx, "cy": b.cy, "cz": b.cz, "vx": b.vx, "vy": b.vy, "vz": b.vz},
-        "autotune": tune,
+        # The stream kernel's population pattern alone (34 shifted directions
+        # pulled and stored, no arithmetic): the ceiling its memory pattern allows.
+        "pattern_probe": head["pattern_probe"],
+        "reference_state_call": head["reference_state_call"],
+        "kernel_ms": head["kernel_ms"],
+        "finite": head["finite"],
+        "sphere": head["sphere"],
+        "autotune": head["autotune"],
     }
-    sim.close()
+    # The compressed form (FP16 deviations) on the same scene and the served
+    # workload in both forms: the metric's ratio and frame latency.
+    if args.storage == "f32" and "f16" in args.legs:
+        f16 = single_leg(args, "f16", copy_gbps)
+        for k in ("sph_r", "params", "sphere"):
+            f16.pop(k)
+        f16["over_f32"] = round(f16["mlups"] / head["mlups"], 3)
+        out["f16"] = f16
+    if "serve" in args.legs and args.serve_seconds > 0:
+        serve = {}
+        for st in ("f32", "f16"):
+            print(f"[bench] serving, {st}, {args.serve_seconds:.0f} s", file=sys.stderr,
+                  flush=True)
+            serve[st] = serve_run(args, st, args.serve_seconds)
+        serve["f16_over_f32"] = round(serve["f16"]["mlups"] / serve["f32"]["mlups"], 3)
+        serve["workload"] = (SERVE_WORKLOAD + f"; {args.serve_seconds:.0f} s per storage form "
+                             "(the configuration's 10 minutes do not fit the benchmark window; "
+                             "--workload serve --duration 600 runs them)")
+        out["serve"] = serve
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(n, args.seed, args.cpu_steps)

@@ -1969,6 +1969,21 @@ __device__ __forceinline__ void body_body(const Scene &sc, int q, int check_band
     ctrl->n_solid = 0;
   }
 }
+// Clears the fresh/dead lists of continued scenes (their last call kept the
+// lists of its last retag readable) before their band retag.
+__device__ __forceinline__ void reset_lists(const Scene &sc) {
+  if (!sc.live) return;
+  sc.ctrl->n_fresh = 0;
+  sc.ctrl->n_dead = 0;
+  sc.ctrl->n_solid = 0;
+}
+__global__ __launch_bounds__(64) void km_reset_lists(const Scene *__restrict__ scenes, int n) {
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  if (j < n) reset_lists(scenes[j]);
+}
+__global__ __launch_bounds__(64) void k1_reset_lists(const Scene sc) {
+  if (threadIdx.x == 0) reset_lists(sc);
+}
 __global__ __launch_bounds__(kBodyThreads) void km_body(const Scene *__restrict__ scenes, int q,
                                                         int check_band, int reset_lists) {
   body_body(scenes[blockIdx.x], q, check_band, reset_lists);

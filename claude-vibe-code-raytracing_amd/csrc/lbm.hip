@@ -280,11 +280,23 @@ static int ensure_f32(lbm_sim *s, hipStream_t st) {
   return LBM_OK;
 }
 
+// After lbm_step_live the state is the fused step's own (post-collision
+// populations for the next step, rho and u stored only near the sphere): entry
+// points that need the reference's state refuse it until an lbm_step call ends
+// in that state.
+#define CHECK_NOT_LIVE(sim)                                                              \
+  do {                                                                                   \
+    if ((sim)->live)                                                                     \
+      return fail((sim), LBM_ERR_STATE,                                                  \
+                  "state between live calls (lbm_step_live): end with lbm_step first"); \
+  } while (0)
+
 // CHECK_SIM for entry points that read or write the float32 populations or
 // anything their compressed baseline depends on.
 #define CHECK_SIM_F32(sim)                                  \
   do {                                                      \
     if (!(sim)) return LBM_ERR_ARG;                         \
+    CHECK_NOT_LIVE(sim);                                    \
     const int rc_f32_ = ensure_f32((sim), (sim)->stream);   \
     if (rc_f32_) return rc_f32_;                            \
   } while (0)
@@ -553,11 +565,13 @@ static int init_columns(lbm_sim *s, const double *yw, double waterline) {
 }
 
 extern "C" int lbm_init_flat(lbm_sim *s, double waterline_y) {
+  if (s) s->live = false;  // initialisation replaces the whole state
   CHECK_SIM_F32(s);
   return init_columns(s, nullptr, waterline_y);
 }
 
 extern "C" int lbm_init_waterline_map(lbm_sim *s, const double *yw) {
+  if (s) s->live = false;  // initialisation replaces the whole state
   CHECK_SIM_F32(s);
   if (!yw) return fail(s, LBM_ERR_ARG, "yw is NULL");
   return init_columns(s, yw, 0.0);
@@ -788,6 +802,7 @@ extern "C" int lbm_sim_step(lbm_sim *s, lbm_sphere *solid) {
 
 extern "C" int lbm_absorb_dead_cells(lbm_sim *s, lbm_sphere *solid) {
   CHECK_SIM(s);
+  CHECK_NOT_LIVE(s);
   if (!solid) return fail(s, LBM_ERR_ARG, "solid is NULL");
   int rc = read_ctrl(s);
   if (rc) return rc;
@@ -860,6 +875,10 @@ static void *field_ptr(lbm_sim *s, int field, int *q, size_t *elem) {
 
 extern "C" int lbm_download(lbm_sim *s, int field, void *host) {
   CHECK_SIM(s);
+  // Between live calls phi, phiPrev, the tags, the surface velocities, the
+  // gradients (recomputed from phiPrev) and the cleared body force are the
+  // reference's; populations, rho and u are not.
+  if (s->live && (field <= LBM_FIELD_UZ)) CHECK_NOT_LIVE(s);
   if (field == LBM_FIELD_F || field == LBM_FIELD_F2 || field == LBM_FIELD_H ||
       field == LBM_FIELD_H2) {
     const int rc = ensure_f32(s, s->stream);
@@ -924,6 +943,7 @@ extern "C" int lbm_upload(lbm_sim *s, int field, const void *host) {
 
 extern "C" int lbm_set_storage(lbm_sim *s, int kind) {
   CHECK_SIM(s);
+  CHECK_NOT_LIVE(s);
   if (kind != LBM_STORAGE_F32 && kind != LBM_STORAGE_F16 && kind != LBM_STORAGE_BF16)
     return fail(s, LBM_ERR_ARG, "unknown storage kind");
   if (kind == s->storage) return LBM_OK;
@@ -949,6 +969,7 @@ extern "C" int lbm_get_storage(const lbm_sim *s) { return s ? s->storage : LBM_E
 
 extern "C" int lbm_mass(lbm_sim *s, double *phi_sum, double *rho_sum) {
   CHECK_SIM(s);
+  CHECK_NOT_LIVE(s);
   const int64_t n = s->g.n;
   const int64_t blocks = (n + 255) / 256;
   int rc = ensure_scratch(s, blocks * 2 * sizeof(double));
@@ -972,6 +993,7 @@ extern "C" int lbm_mass(lbm_sim *s, double *phi_sum, double *rho_sum) {
 
 extern "C" int lbm_check_fields(lbm_sim *s, int *all_finite, double *max_u) {
   CHECK_SIM(s);
+  CHECK_NOT_LIVE(s);
   const int64_t n = s->g.n;
   const int64_t blocks = (n + 255) / 256;
   int rc = ensure_scratch(s, blocks * 2 * sizeof(double));

@@ -88,7 +88,7 @@ class ServerStats(C.Structure):
 class TuneResult(C.Structure):
     """lbm_tune_result: the launch configuration the autotuner chose."""
     _fields_ = [("cells_per_lane", C.c_int), ("graph_steps", C.c_int),
-                ("ms_per_step", C.c_double), ("candidates", C.c_int)]
+                ("ms_per_step", C.c_double), ("candidates", C.c_int), ("stream_split", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -134,6 +134,7 @@ def lib():
         "lbm_absorb_dead_cells": (C.c_int, [P, SP]),
         "lbm_step": (C.c_int, [P, SP, C.c_int]),
         "lbm_step_noalloc": (C.c_int, [P, SP, C.c_int]),
+        "lbm_step_live": (C.c_int, [P, SP, C.c_int]),
         "lbm_set_stream_split": (C.c_int, [P, C.c_int]),
         "lbm_field_count": (C.c_int64, [P, C.c_int]),
         "lbm_download": (C.c_int, [P, C.c_int, P]),
@@ -360,6 +361,12 @@ class Sim:
     def coupled(self, b, nsteps):
         """nsteps fused coupled steps (lbm_step)."""
         self._ck(self.L.lbm_step(self.h, self._sp(b), int(nsteps)), "lbm_step")
+
+    def coupled_live(self, b, nsteps):
+        """lbm_step_live: steps that end live (the next call continues without
+        the per-call prologue); only phi, the tags, lists and sphere are the
+        reference's until a coupled() call ends the run."""
+        self._ck(self.L.lbm_step_live(self.h, self._sp(b), int(nsteps)), "lbm_step_live")
 
     def coupled_noalloc(self, b, nsteps):
         """lbm_step_noalloc: the same steps without device allocation."""

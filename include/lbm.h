@@ -182,6 +182,23 @@ int lbm_absorb_dead_cells(lbm_sim *sim, lbm_sphere *solid);
  * per-pass path in the last bits.  solid may be NULL (no body). */
 int lbm_step(lbm_sim *sim, lbm_sphere *solid, int nsteps);
 
+/* nsteps coupled steps like lbm_step, ending live: the call stops after the
+ * last step's sphere update and defers that step's band work (the next
+ * step's retag, fresh cells and collide near the sphere) to the next call.  No
+ * reference counterpart.  A call that starts from the reference's state runs
+ * a prologue (a retag, fresh-cell pass and collide of every cell) and one that
+ * ends in it a stream-only last step; live calls skip both, so consecutive
+ * lbm_step_live calls cost what one long call costs -- a frame loop that steps
+ * two substeps per frame (src/worker.js:94-103) pays no per-call passes.
+ * Between live calls phi, phiPrev, the tags, the surface velocities, the
+ * gradients, the fresh/dead lists and the sphere are the reference's after
+ * those steps (lbm_pack_volume works); the populations, rho and u are not, and
+ * every entry point that needs them (their download, any upload, the per-pass
+ * calls, lbm_set_storage, lbm_mass, lbm_check_fields) returns LBM_ERR_STATE
+ * until an lbm_step call of at least one step has ended in the reference's
+ * state.  lbm_init_flat / lbm_init_waterline_map start over. */
+int lbm_step_live(lbm_sim *sim, lbm_sphere *solid, int nsteps);
+
 /* lbm_step that allocates no device memory.  No reference counterpart: it is
  * for callers that cannot allocate (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  Same kernels and results as
@@ -249,8 +266,10 @@ int lbm_batch_set_stream_lanes(lbm_batch *batch, int cells_per_lane);
 
 /* Launch-shape autotuner.  No reference counterpart.  Times every candidate
  * launch configuration of the batch -- cells per lane of the 16-bit stream
- * kernel (when a member stores 16-bit populations) and steps per captured
- * graph (0, 2, 8, 32, those below trial_steps - 1) -- over trial_steps real
+ * kernel (when a member stores 16-bit populations), for one float32 simulation
+ * (lbm_autotune) the stream kernel's bulk/band split (lbm_set_stream_split),
+ * and steps per captured graph (0, 2, 8, 32, those below trial_steps - 1) --
+ * over trial_steps real
  * coupled steps each, after an untimed lead-in of 2 steps, keeps the fastest
  * and reports it.  Every candidate computes the same state, so the trials
  * simply advance the members: after the call each has advanced by
@@ -261,6 +280,7 @@ typedef struct {
   int graph_steps;     /* 0: kernels launched one by one */
   double ms_per_step;  /* of the chosen configuration */
   int candidates;      /* configurations timed */
+  int stream_split;    /* lbm_set_stream_split of the choice (float32, one simulation) */
 } lbm_tune_result;
 int lbm_batch_autotune(lbm_batch *batch, int trial_steps, lbm_tune_result *out);
 /* The same for one simulation's lbm_step. */
