@@ -11,30 +11,45 @@ namespace lbm {
 // Grid geometry.  Cell arrays (phi, rho, tags, ...) hold cell i = (z*ny + y)*nx + x
 // at i.  S = n rounded up to 64 cells (256 bytes) is the plane stride of the
 // 16-bit population codecs, which keep population k of cell i at k*S + i.
+//
+// Float32 populations: one buffer per step parity holds the 27 f directions
+// and then the 7 h directions (the h pointer is the f pointer plus Q27*pb);
+// population k of cell i (row r = i / nx = z*ny + y) is element
+//   i + r*pa + k*pb
+// of its pointer.  Two layouts:
+//  * row blocks (pa = 33 nx, pb = nx): every grid row's 34 directions side by
+//    side, element (r*34 + k)*nx + x.  A lane's pulls land in the 34 row chunks
+//    of three neighbouring row blocks instead of 34 planes a whole grid apart:
+//    the 34-direction pull-and-store pattern runs at 5.6 TB/s against 5.25 for
+//    planes at 256^3 on MI355X (5.6 against 5.1 at 384^3);
+//  * planes (pa = 0, pb = S): direction k of every cell, then the next.  Narrow
+//    grids run faster so: a wave's 64 cells then span several rows, which row
+//    blocks scatter (64 scenes of 32..96 cells per axis stepped together: 8.96 k
+//    MLUPS in row blocks against 10.5 k in planes).
+// lbm_create chooses row blocks when nx >= kRowLayoutMinNx.  Either way a
+// direction's source element is a uniform offset from the cell's own.
 struct Geom {
   int nx, ny, nz, nxny;
   int64_t n, S;
+  int64_t pa, pb;  // float32 population layout (above)
 };
 
-// Float32 populations are stored in row blocks: one buffer per step parity
-// holds, for every grid row r = z*ny + y in order, the row's 27 f directions
-// and then its 7 h directions, nx cells each.  Population k of cell
-// i = r*nx + x is element (r*kRowQ + k)*nx + x of the f pointer; h's
-// direction k is the same element of the h pointer, which is the f pointer
-// plus Q27*nx.  A lane's pulls then land in the 34 row chunks of three
-// neighbouring row blocks instead of 34 planes a whole grid apart: the 34-plane
-// pull-and-store pattern runs at 5.6 TB/s against 5.25 TB/s for planes at
-// 256^3 on MI355X (5.6 against 5.1 at 384^3), and a direction's source is still
-// a uniform offset from the cell's own element.
-constexpr int kRowQ = Q27 + Q7;
-// Element of direction 0 of cell i in a row-block buffer.
-__host__ __device__ inline int64_t row_elem(const Geom &g, int64_t i) {
-  return i + (i / g.nx) * (int64_t)(kRowQ - 1) * g.nx;
+constexpr int kRowQ = 27 + 7;
+constexpr int kRowLayoutMinNx = 128;
+enum : int { LAYOUT_PLANES = 0, LAYOUT_ROWS = 1 };
+inline void set_layout(Geom &g, int layout) {
+  g.pa = layout == LAYOUT_ROWS ? (int64_t)(kRowQ - 1) * g.nx : 0;
+  g.pb = layout == LAYOUT_ROWS ? (int64_t)g.nx : g.S;
+}
+// Element of direction 0 of cell i in a float32 population buffer.
+__host__ __device__ inline int64_t pop_base(const Geom &g, int64_t i) {
+  return g.pa ? i + (i / g.nx) * g.pa : i;
 }
 // Offset from a cell's element of direction k to its source's element of the
-// same direction: the source is x - ex, y - ey, z - ez.
-__host__ __device__ constexpr int64_t row_src_delta(int nx, int ny, int ex, int ey, int ez) {
-  return ((int64_t)ez * ny + ey) * kRowQ * nx + ex;
+// same direction: the source is x - ex, y - ey, z - ez, that is (ez*ny + ey)
+// rows and ex cells before.
+__host__ __device__ inline int64_t pop_src_delta(const Geom &g, int ex, int ey, int ez) {
+  return ((int64_t)ez * g.ny + ey) * (g.nx + g.pa) + ex;
 }
 
 // Per-step constants, computed on the host from lbm_params with the same
@@ -93,10 +108,12 @@ struct TileMap {
 
 constexpr int kTileShift = 8;  // 256 cells per tile: one cell per lane
 // The fused float32 kernels address a direction's two pull candidates by a
-// 32-bit byte offset from a uniform base, which spans about (ny + 2) row
-// blocks: true when a grid's row blocks allow it.
-inline bool f32_fused_fits(int nx, int ny) {
-  return (int64_t)(ny + 2) * kRowQ * nx * 4 + (1 << 20) < ((int64_t)1 << 31);
+// 32-bit byte offset from a uniform base: a wave's own span (at most three rows),
+// one direction stride and a source delta (ny + 1 rows).  True when that stays
+// below 2^31 bytes.
+inline bool f32_fused_fits(const Geom &g) {
+  const int64_t span = 64 + 3 * (g.nx + g.pa) + g.pb + (int64_t)(g.ny + 1) * (g.nx + g.pa) + 2;
+  return span * 4 < ((int64_t)1 << 31);
 }
 
 inline TileMap make_tile_map(int nx, int ny, int nz) {

@@ -469,36 +469,39 @@ __device__ __forceinline__ T pull_elem(const T *__restrict__ p2, int64_t S, int 
   return ldb(base, bounce ? own : src);
 }
 
-// Float32 populations (row blocks) of the cells of one tile of 256 consecutive
-// cells: a wave-uniform element base (row_elem of the tile's first cell) and
-// each lane's 32-bit byte offset from it to its own cell's direction-0 element,
-// so that every load and store adds one per-lane offset to a uniform base and
-// needs no 64-bit address arithmetic per lane.
+// Float32 populations of the cells of one wave: a wave-uniform element base
+// and each lane's 32-bit byte offset from it to its own cell's direction-0
+// element, so that every load and store adds one per-lane offset to a uniform
+// base and needs no 64-bit address arithmetic per lane.
 struct RowAddr {
   int64_t base;
   uint32_t relb;
 };
-// i0: the tile's first cell; i, r: the lane's cell and its row (z ny + y).
-__device__ __forceinline__ RowAddr row_addr(const Geom &g, int64_t i0, int64_t i, int r) {
-  const int r0 = (int)(i0 / g.nx);
+// i, r: the lane's cell and its row (z ny + y).  The base is the element of the
+// wave's first active lane (the lowest cell, in the lowest row, among the
+// active lanes), read from that lane, so nothing is divided.
+__device__ __forceinline__ RowAddr row_addr(const Geom &g, int64_t i, int r) {
+  const int ii = (int)i;
+  const int i0 = __builtin_amdgcn_readfirstlane(ii);
+  const int r0 = __builtin_amdgcn_readfirstlane(r);
+  const int pa = (int)g.pa;
   RowAddr a;
-  a.base = i0 + (int64_t)r0 * (kRowQ - 1) * g.nx;
-  a.relb = (uint32_t)((i - i0) + (int64_t)(r - r0) * (kRowQ - 1) * g.nx) * 4u;
+  a.base = (int64_t)i0 + (int64_t)r0 * pa;
+  a.relb = (uint32_t)((ii - i0) + (r - r0) * pa) * 4u;
   return a;
 }
 // Direction k (opposite kopp, source (x - ex, y - ey, z - ez)) of a cell, from
-// the row-block pointer p (the f pointer, or the h pointer for D3Q7): the
+// the population pointer p (the f pointer, or the h pointer for D3Q7): the
 // source's element, or at a non-fluid source the cell's own element of the
 // opposite direction.  Both candidates are uniform offsets from the lane's own
 // element; the uniform base is the lower of them, and the per-lane choice is a
-// 32-bit byte offset.  Needs the plane span (ny + 2) kRowQ nx of 4-byte elements
-// below 2^31 bytes (checked on the host), and the source inside the grid, which
-// holds for fluid cells.
+// 32-bit byte offset.  Needs f32_fused_fits (checked on the host) and the
+// source inside the grid, which holds for fluid cells.
 __device__ __forceinline__ float pull_row(const float *__restrict__ p, const RowAddr &a,
                                           const Geom &g, int k, int kopp, int ex, int ey, int ez,
                                           bool bounce) {
-  const int64_t cs = (int64_t)k * g.nx - row_src_delta(g.nx, g.ny, ex, ey, ez);
-  const int64_t co = (int64_t)kopp * g.nx;
+  const int64_t cs = k * g.pb - pop_src_delta(g, ex, ey, ez);
+  const int64_t co = kopp * g.pb;
   const int64_t lo = cs < co ? cs : co;
   const float *b = p + a.base + lo;
   const uint32_t ds = (uint32_t)(cs - lo) * 4u, dop = (uint32_t)(co - lo) * 4u;
@@ -831,16 +834,16 @@ __device__ __forceinline__ void collide_body(const Scene &sc, int lb, int p) {
     return;
   }
   // Float32: row blocks.
-  const int64_t pe = row_elem(g, i);
+  const int64_t pe = pop_base(g, i);
   float hin[Q7], hc[Q7];
 #pragma unroll
-  for (int k = 0; k < Q7; k++) hin[k] = C::ldo(hi + pe + (int64_t)k * g.nx, 0u, w7(k) * phc);
+  for (int k = 0; k < Q7; k++) hin[k] = C::ldo(hi + pe + k * g.pb, 0u, w7(k) * phc);
   collide_phase_cell<R>(sc.d, pc, uxe, uye, uze, (R)gx1, (R)gy1, (R)gz1, hin, hc);
 #pragma unroll
-  for (int k = 0; k < Q7; k++) C::sto(ho + pe + (int64_t)k * g.nx, 0u, hc[k], w7(k) * phc);
+  for (int k = 0; k < Q7; k++) C::sto(ho + pe + k * g.pb, 0u, hc[k], w7(k) * phc);
   float fin[Q27], fc[Q27];
 #pragma unroll
-  for (int k = 0; k < Q27; k++) fin[k] = C::ldo(fi + pe + (int64_t)k * g.nx, 0u, w27(k));
+  for (int k = 0; k < Q27; k++) fin[k] = C::ldo(fi + pe + k * g.pb, 0u, w27(k));
   const bool bf = !BAND && sc.use_bf;
   const R bx = bf ? (R)sc.bfx[i] : R(0);
   const R by = bf ? (R)sc.bfy[i] : R(0);
@@ -848,7 +851,7 @@ __device__ __forceinline__ void collide_body(const Scene &sc, int lb, int p) {
   collide_hydro_cell<R>(sc.d, (R)sc.rho[i], pc, (R)lap1, (R)gx1, (R)gy1, (R)gz1, bx, by, bz, uxe,
                         uye, uze, fin, fc);
 #pragma unroll
-  for (int k = 0; k < Q27; k++) C::sto(fo + pe + (int64_t)k * g.nx, 0u, fc[k], w27(k));
+  for (int k = 0; k < Q27; k++) C::sto(fo + pe + k * g.pb, 0u, fc[k], w27(k));
 }
 template <bool BAND, int KIND>
 __global__ __launch_bounds__(256) void km_collide(const Scene *__restrict__ scenes,
@@ -922,7 +925,7 @@ __device__ __forceinline__ void phase_stream_body(const Scene &sc, int lb, int q
     stb(phi_next, bo4, phase_sum_dev(e, phib, hfull));
   } else {
     float hv[Q7];
-    pull7<KIND>(g, row_addr(g, (int64_t)tile << kTileShift, i, z * g.ny + y), h2, nonfluid, hv);
+    pull7<KIND>(g, row_addr(g, i, z * g.ny + y), h2, nonfluid, hv);
     R phiSum = 0;
 #pragma unroll
     for (int k = 0; k < Q7; k++) phiSum += (R)hv[k];
@@ -1253,7 +1256,7 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
         nonfluid = wall_mask(g, x, y, z);
         solid = 0;
       }
-      const RowAddr ra = row_addr(g, (int64_t)tile << kTileShift, i, z * g.ny + y);
+      const RowAddr ra = row_addr(g, i, z * g.ny + y);
       float fp[Q27];
       R pSumF, mx, my, mz;
       pull27<KIND>(g, i, ra, f2, nonfluid, solid, sc.usx, sc.usy, sc.usz, rhoPhaseOld, exchange,
@@ -1263,8 +1266,8 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
       T *fob = fo + ra.base, *hob = ho + ra.base;
       finish_cell<COLLIDE, KIND>(
           sc, i, near, phi_next, gxo, gyo, gzo, lapo, fp, hp, pSumF, mx, my, mz, use_bf, phi_next,
-          [&](int k, float val, float base) { C::sto(fob + (int64_t)k * g.nx, ra.relb, val, base); },
-          [&](int k, float val, float base) { C::sto(hob + (int64_t)k * g.nx, ra.relb, val, base); });
+          [&](int k, float val, float base) { C::sto(fob + k * g.pb, ra.relb, val, base); },
+          [&](int k, float val, float base) { C::sto(hob + k * g.pb, ra.relb, val, base); });
     }
   }
   if (exchange) tile_momentum(mom, sc.partials, tile);
@@ -1737,11 +1740,11 @@ __global__ __launch_bounds__(256) void k_encode(Geom g, const float *__restrict_
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= g.n) return;
   const float pc = phi[i];
-  const int64_t pe = row_elem(g, i), S = g.S;
+  const int64_t pe = pop_base(g, i), S = g.S;
 #pragma unroll
-  for (int k = 0; k < Q27; k++) C::st(cf, k * S + i, f[pe + (int64_t)k * g.nx], w27(k));
+  for (int k = 0; k < Q27; k++) C::st(cf, k * S + i, f[pe + k * g.pb], w27(k));
 #pragma unroll
-  for (int k = 0; k < Q7; k++) C::st(ch, k * S + i, h[pe + (int64_t)k * g.nx], w7(k) * pc);
+  for (int k = 0; k < Q7; k++) C::st(ch, k * S + i, h[pe + k * g.pb], w7(k) * pc);
 }
 template <int KIND>
 __global__ __launch_bounds__(256) void k_decode(Geom g,
@@ -1753,11 +1756,11 @@ __global__ __launch_bounds__(256) void k_decode(Geom g,
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= g.n) return;
   const float pc = phi[i];
-  const int64_t pe = row_elem(g, i), S = g.S;
+  const int64_t pe = pop_base(g, i), S = g.S;
 #pragma unroll
-  for (int k = 0; k < Q27; k++) f[pe + (int64_t)k * g.nx] = C::ld(cf, k * S + i, w27(k));
+  for (int k = 0; k < Q27; k++) f[pe + k * g.pb] = C::ld(cf, k * S + i, w27(k));
 #pragma unroll
-  for (int k = 0; k < Q7; k++) h[pe + (int64_t)k * g.nx] = C::ld(ch, k * S + i, w7(k) * pc);
+  for (int k = 0; k < Q7; k++) h[pe + k * g.pb] = C::ld(ch, k * S + i, w7(k) * pc);
 }
 
 // Global mass check: sums over interior fluid cells of phi and of rho (the

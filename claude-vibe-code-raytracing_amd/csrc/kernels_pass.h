@@ -17,7 +17,7 @@ namespace lbm {
 // planes of stride S for the 16-bit codecs.
 template <int KIND>
 __device__ __forceinline__ int64_t pop_elem(const Geom &g, int64_t i, int k) {
-  if constexpr (KIND == ST_F32) return row_elem(g, i) + (int64_t)k * g.nx;
+  if constexpr (KIND == ST_F32) return pop_base(g, i) + k * g.pb;
   else return (int64_t)k * g.S + i;
 }
 
@@ -47,11 +47,11 @@ __global__ __launch_bounds__(256) void k_init_cells(Geom g, const double *__rest
   const bool onWall = x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1;
   tag[i] = onWall ? TAG_WALL : TAG_FLUID;
   tagPrev[i] = tag[i];
-  const int64_t pe = row_elem(g, i);
+  const int64_t pe = pop_base(g, i);
 #pragma unroll
-  for (int k = 0; k < Q27; k++) f[pe + (int64_t)k * g.nx] = w27(k);
+  for (int k = 0; k < Q27; k++) f[pe + k * g.pb] = w27(k);
 #pragma unroll
-  for (int k = 0; k < Q7; k++) h[pe + (int64_t)k * g.nx] = (float)((double)w7(k) * pc);
+  for (int k = 0; k < Q7; k++) h[pe + k * g.pb] = (float)((double)w7(k) * pc);
 }
 
 // seedHydrostatic (src/sim.js:144-173): one thread per (x,z) column, sequential
@@ -70,14 +70,14 @@ __global__ __launch_bounds__(256) void k_seed_hydrostatic(Geom g, double rhoG, d
   double pCs2 = 1.0;
   for (int y = 0; y < g.ny; y++) {
     const int64_t i = cell_index(g, x, y, z);
-    const int64_t pe = row_elem(g, i);
+    const int64_t pe = pop_base(g, i);
     rho[i] = (float)pCs2;
     const double rhoPhase = rhoG + (double)phi[i] * dRho;
 #pragma unroll
     for (int k = 0; k < Q27; k++) {
       const double eu = (double)EY27[k] * uLatY;
       const double Qi = eu * (INV_CS2_27 + INV_2CS4_27 * eu) - 1.5 * uLat2;
-      f[pe + (int64_t)k * g.nx] = (float)((double)w27(k) * (pCs2 + rhoPhase * Qi));
+      f[pe + k * g.pb] = (float)((double)w27(k) * (pCs2 + rhoPhase * Qi));
     }
     pCs2 += rhoPhase * gOverCs2;
   }
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(256) void k_collide_phase(Geom g, Derived d,
   const double Sx = sharpen * d.sharpFac * ggx / gmag;
   const double Sy = sharpen * d.sharpFac * ggy / gmag;
   const double Sz = sharpen * d.sharpFac * ggz / gmag;
-  const int64_t pe = row_elem(g, i);
+  const int64_t pe = pop_base(g, i);
 #pragma unroll
   for (int k = 0; k < Q7; k++) {
     const double ekx = EX7[k], eky = EY7[k], ekz = EZ7[k];
@@ -398,8 +398,8 @@ __global__ __launch_bounds__(256) void k_collide_phase(Geom g, Derived d,
     const double eu = ekx * uxi + eky * uyi + ekz * uzi;
     const double heq = wk * p * (1 + INV_CS2_7 * eu);
     const double Fk = wk * (ekx * Sx + eky * Sy + ekz * Sz);
-    const double hk = h[pe + (int64_t)k * g.nx];
-    h2[pe + (int64_t)k * g.nx] = (float)(hk - d.invTauPhi * (hk - heq) + Fk);
+    const double hk = h[pe + k * g.pb];
+    h2[pe + k * g.pb] = (float)(hk - d.invTauPhi * (hk - heq) + Fk);
   }
 }
 
@@ -411,12 +411,12 @@ __global__ __launch_bounds__(256) void k_stream_phase(Geom g, const float *__res
   if (!interior(g, x, y, z)) return;
   const int64_t i = cell_index(g, x, y, z);
   if (tag[i] != TAG_FLUID) return;
-  const int64_t pe = row_elem(g, i);
+  const int64_t pe = pop_base(g, i);
 #pragma unroll
   for (int k = 0; k < Q7; k++) {
     const int64_t s = i - ((int64_t)EZ7[k] * g.nxny + (int64_t)EY7[k] * g.nx + EX7[k]);
-    h[pe + (int64_t)k * g.nx] = tag[s] == TAG_FLUID ? h2[row_elem(g, s) + (int64_t)k * g.nx]
-                                                    : h2[pe + (int64_t)opp7(k) * g.nx];
+    h[pe + k * g.pb] = tag[s] == TAG_FLUID ? h2[pop_base(g, s) + k * g.pb]
+                                                    : h2[pe + opp7(k) * g.pb];
   }
 }
 
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(256) void k_collide_hydro(
   const double u2 = uxe * uxe + uye * uye + uze * uze;
   const double negU2half = -1.5 * u2;
   const double uF = uxe * fx + uye * fy + uze * fz;
-  const int64_t pe = row_elem(g, i);
+  const int64_t pe = pop_base(g, i);
 #pragma unroll
   for (int k = 0; k < Q27; k++) {
     const double ekx = EX27[k], eky = EY27[k], ekz = EZ27[k];
@@ -456,8 +456,8 @@ __global__ __launch_bounds__(256) void k_collide_hydro(
     const double feq = wk * (sumF + rhoPhase * Qi);
     const double ekF = ekx * fx + eky * fy + ekz * fz;
     const double Sk = d.A * wk * (ekF * (1 + INV_CS2_27 * eu) - uF);
-    const double fk = f[pe + (int64_t)k * g.nx];
-    f2[pe + (int64_t)k * g.nx] = (float)(fk - d.invTau * (fk - feq) + Sk);
+    const double fk = f[pe + k * g.pb];
+    f2[pe + k * g.pb] = (float)(fk - d.invTau * (fk - feq) + Sk);
   }
 }
 
@@ -478,7 +478,7 @@ __global__ __launch_bounds__(256) void k_stream_hydro_macro(
   const int64_t i = cell_index(g, x, y, z);
   if (tag[i] != TAG_FLUID) return;
   const double rhoPhaseOld = d.rhoG + (double)phi[i] * d.dRho;
-  const int64_t pe = row_elem(g, i);
+  const int64_t pe = pop_base(g, i);
   double pSumF = 0, mx = 0, my = 0, mz = 0;
 #pragma unroll
   for (int k = 0; k < Q27; k++) {
@@ -487,9 +487,9 @@ __global__ __launch_bounds__(256) void k_stream_hydro_macro(
     const double ekx = EX27[k], eky = EY27[k], ekz = EZ27[k];
     double fk;
     if (st == TAG_FLUID) {
-      fk = f2[row_elem(g, s) + (int64_t)k * g.nx];
+      fk = f2[pop_base(g, s) + k * g.pb];
     } else {
-      const double fopp = f2[pe + (int64_t)opp27(k) * g.nx];
+      const double fopp = f2[pe + opp27(k) * g.pb];
       double corr = 0;
       if (st == TAG_SOLID) {
         const double eu = ekx * (double)usx[s] + eky * (double)usy[s] + ekz * (double)usz[s];
@@ -507,7 +507,7 @@ __global__ __launch_bounds__(256) void k_stream_hydro_macro(
         }
       }
     }
-    f[pe + (int64_t)k * g.nx] = (float)fk;
+    f[pe + k * g.pb] = (float)fk;
     pSumF += fk;
     mx += ekx * fk;
     my += eky * fk;
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(256) void k_stream_hydro_macro(
   rho[i] = (float)(pSumF > 0.01 ? pSumF : 0.01);
   double phiSum = 0;
 #pragma unroll
-  for (int k = 0; k < Q7; k++) phiSum += (double)h[pe + (int64_t)k * g.nx];
+  for (int k = 0; k < Q7; k++) phiSum += (double)h[pe + k * g.pb];
   const double p = phiSum < 0 ? 0 : phiSum > 1 ? 1 : phiSum;
   phi[i] = (float)p;
   const double rhoPhaseNew = d.rhoG + p * d.dRho;
@@ -625,15 +625,15 @@ __global__ __launch_bounds__(256) void k_aos_to_rows(Geom g, int q, const float 
                                                      float *__restrict__ rows) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.n) return;
-  const int64_t pe = row_elem(g, i);
-  for (int k = 0; k < q; k++) rows[pe + (int64_t)k * g.nx] = aos[i * q + k];
+  const int64_t pe = pop_base(g, i);
+  for (int k = 0; k < q; k++) rows[pe + k * g.pb] = aos[i * q + k];
 }
 __global__ __launch_bounds__(256) void k_rows_to_aos(Geom g, int q, const float *__restrict__ rows,
                                                      float *__restrict__ aos) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.n) return;
-  const int64_t pe = row_elem(g, i);
-  for (int k = 0; k < q; k++) aos[i * q + k] = rows[pe + (int64_t)k * g.nx];
+  const int64_t pe = pop_base(g, i);
+  for (int k = 0; k < q; k++) aos[i * q + k] = rows[pe + k * g.pb];
 }
 
 }  // namespace lbm

@@ -409,6 +409,7 @@ extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
   s->g.nxny = nx * ny;
   s->g.n = n;
   s->g.S = (n + 63) / 64 * 64;
+  set_layout(s->g, nx >= kRowLayoutMinNx ? LAYOUT_ROWS : LAYOUT_PLANES);
   hipGetDevice(&s->device);
   scratch_setup(s->device);
   // src/sim.js:88-95 defaults.
@@ -422,8 +423,8 @@ extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
   if (e == hipSuccess) {                                         \
     e = dev_malloc((void **)&(ptr), (bytes));                     \
   }
-  ALLOC(s->fbuf[0], n * kRowQ * sizeof(float));
-  ALLOC(s->fbuf[1], n * kRowQ * sizeof(float));
+  ALLOC(s->fbuf[0], S * kRowQ * sizeof(float));
+  ALLOC(s->fbuf[1], S * kRowQ * sizeof(float));
   ALLOC(s->phibuf[0], S * sizeof(float));
   ALLOC(s->phibuf[1], S * sizeof(float));
   ALLOC(s->rho, S * sizeof(float));
@@ -448,14 +449,14 @@ extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
   ALLOC(s->ctrl, sizeof(Ctrl));
 #undef ALLOC
   if (e == hipSuccess)
-    for (int b = 0; b < 2; b++) s->hbuf[b] = s->fbuf[b] + (int64_t)Q27 * nx;
+    for (int b = 0; b < 2; b++) s->hbuf[b] = s->fbuf[b] + Q27 * s->g.pb;
   if (e == hipSuccess) e = host_malloc((void **)&s->ctrl_host, sizeof(Ctrl), hipHostMallocDefault);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e == hipSuccess) {
     // Zero the whole state so reads before an init are defined.
     const int64_t fb = S * sizeof(float);
     for (int b = 0; b < 2; b++) {
-      hipMemsetAsync(s->fbuf[b], 0, n * kRowQ * sizeof(float), s->stream);
+      hipMemsetAsync(s->fbuf[b], 0, S * kRowQ * sizeof(float), s->stream);
       hipMemsetAsync(s->phibuf[b], 0, fb, s->stream);
     }
     float *arrs[] = {s->rho, s->ux, s->uy, s->uz, s->usx, s->usy, s->usz, s->gx, s->gy, s->gz,
@@ -1198,7 +1199,7 @@ __global__ __launch_bounds__(256) void k_pull_probe32(const float *__restrict__ 
   if (i >= g.n) return;
   int x, y, z;
   cell_xyz(g.nx, g.nxny, i, x, y, z);
-  const RowAddr ra = row_addr(g, (int64_t)tile << kTileShift, i, z * g.ny + y);
+  const RowAddr ra = row_addr(g, i, z * g.ny + y);
   float v[Q27 + Q7];
 #pragma unroll
   for (int k = 0; k < Q27 + Q7; k++) {
@@ -1212,7 +1213,7 @@ __global__ __launch_bounds__(256) void k_pull_probe32(const float *__restrict__ 
     v[k] = pull_row(src, ra, g, k, kopp, ex, ey, ez, !in);
   }
 #pragma unroll
-  for (int k = 0; k < Q27 + Q7; k++) stn(dst + ra.base + (int64_t)k * g.nx, ra.relb, v[k]);
+  for (int k = 0; k < Q27 + Q7; k++) stn(dst + ra.base + k * g.pb, ra.relb, v[k]);
 }
 __global__ __launch_bounds__(128) void k_pull_probe16(const uint16_t *__restrict__ src,
                                                       uint16_t *__restrict__ dst, Geom g,
@@ -1260,9 +1261,10 @@ extern "C" int lbm_probe_pull_bandwidth(int nx, int ny, int nz, int storage, int
   g.nxny = nx * ny;
   g.n = (int64_t)g.nxny * nz;
   g.S = (g.n + 63) / 64 * 64;
+  set_layout(g, nx >= kRowLayoutMinNx ? LAYOUT_ROWS : LAYOUT_PLANES);  // as lbm_create
   const int elem = storage == LBM_STORAGE_F32 ? 4 : 2;
   const int64_t bytes = (int64_t)(Q27 + Q7) * g.S * elem;
-  if (elem == 4 ? !f32_fused_fits(nx, ny) : bytes >= ((int64_t)1 << 32)) return LBM_ERR_ARG;
+  if (elem == 4 ? !f32_fused_fits(g) : bytes >= ((int64_t)1 << 32)) return LBM_ERR_ARG;
   const TileMap tm = make_tile_map(nx, ny, nz);
   void *a = nullptr, *b = nullptr;
   hipStream_t st = nullptr;
