@@ -26,8 +26,9 @@ namespace lbm {
 //    grids run faster so: a wave's 64 cells then span several rows, which row
 //    blocks scatter (64 scenes of 32..96 cells per axis stepped together: 8.96 k
 //    MLUPS in row blocks against 10.5 k in planes).
-// lbm_create chooses row blocks when nx >= kRowLayoutMinNx.  Either way a
-// direction's source element is a uniform offset from the cell's own.
+// lbm_create chooses row blocks when nx >= kRowLayoutMinNx (or when planes
+// would not fit the fused kernels' 32-bit offsets, f32_fused_fits).  Either way
+// a direction's source element is a uniform offset from the cell's own.
 struct Geom {
   int nx, ny, nz, nxny;
   int64_t n, S;
@@ -107,13 +108,24 @@ struct TileMap {
 };
 
 constexpr int kTileShift = 8;  // 256 cells per tile: one cell per lane
-// The fused float32 kernels address a direction's two pull candidates by a
-// 32-bit byte offset from a uniform base: a wave's own span (at most three rows),
-// one direction stride and a source delta (ny + 1 rows).  True when that stays
-// below 2^31 bytes.
+// D of the fused float32 pulls: the largest distance of a source element below
+// a cell's own element of the same direction ((ny + 1) rows and one cell).
+__host__ __device__ inline int64_t pull_span(const Geom &g) {
+  return (int64_t)(g.ny + 1) * (g.nx + g.pa) + 1;
+}
+// The fused float32 kernels address every population of a wave by a 32-bit
+// byte offset from one base per buffer: a lane's offset (within three rows),
+// up to 33 direction strides and twice D.  True when that stays below 2^31
+// bytes (so element offsets also fit 32 bits).
 inline bool f32_fused_fits(const Geom &g) {
-  const int64_t span = 64 + 3 * (g.nx + g.pa) + g.pb + (int64_t)(g.ny + 1) * (g.nx + g.pa) + 2;
+  const int64_t span = 64 + 3 * (g.nx + g.pa) + (kRowQ - 1) * g.pb + 2 * pull_span(g);
   return span * 4 < ((int64_t)1 << 31);
+}
+// The layout lbm_create gives a grid (g.S set): row blocks from nx >= 128, or
+// when planes would not fit the fused kernels' offsets.
+inline void choose_layout(Geom &g) {
+  set_layout(g, LAYOUT_PLANES);
+  if (g.nx >= kRowLayoutMinNx || !f32_fused_fits(g)) set_layout(g, LAYOUT_ROWS);
 }
 
 inline TileMap make_tile_map(int nx, int ny, int nz) {

@@ -472,7 +472,10 @@ __device__ __forceinline__ T pull_elem(const T *__restrict__ p2, int64_t S, int 
 // Float32 populations of the cells of one wave: a wave-uniform element base
 // and each lane's 32-bit byte offset from it to its own cell's direction-0
 // element, so that every load and store adds one per-lane offset to a uniform
-// base and needs no 64-bit address arithmetic per lane.
+// base and needs no 64-bit address arithmetic per lane.  Pulls use one base per
+// buffer, D elements below the wave's own (D: the largest distance of a source
+// element below a cell's own, pull_span), so every direction's two candidates
+// are 32-bit byte offsets above it.
 struct RowAddr {
   int64_t base;
   uint32_t relb;
@@ -490,22 +493,24 @@ __device__ __forceinline__ RowAddr row_addr(const Geom &g, int64_t i, int r) {
   a.relb = (uint32_t)((ii - i0) + (r - r0) * pa) * 4u;
   return a;
 }
-// Direction k (opposite kopp, source (x - ex, y - ey, z - ez)) of a cell, from
-// the population pointer p (the f pointer, or the h pointer for D3Q7): the
-// source's element, or at a non-fluid source the cell's own element of the
-// opposite direction.  Both candidates are uniform offsets from the lane's own
-// element; the uniform base is the lower of them, and the per-lane choice is a
-// 32-bit byte offset.  Needs f32_fused_fits (checked on the host) and the
-// source inside the grid, which holds for fluid cells.
-__device__ __forceinline__ float pull_row(const float *__restrict__ p, const RowAddr &a,
+// Direction k (opposite kopp, source (x - ex, y - ey, z - ez)) of a cell from
+// lo = the population pointer plus the wave base minus D elements: the source's
+// element, or at a non-fluid source the cell's own element of the opposite
+// direction; the per-lane choice is between two uniform 32-bit offsets.  Needs
+// f32_fused_fits (checked on the host) and the source inside the grid, which
+// holds for fluid cells.
+__device__ __forceinline__ float pull_row(const float *__restrict__ lo, uint32_t relb,
                                           const Geom &g, int k, int kopp, int ex, int ey, int ez,
                                           bool bounce) {
-  const int64_t cs = k * g.pb - pop_src_delta(g, ex, ey, ez);
-  const int64_t co = kopp * g.pb;
-  const int64_t lo = cs < co ? cs : co;
-  const float *b = p + a.base + lo;
-  const uint32_t ds = (uint32_t)(cs - lo) * 4u, dop = (uint32_t)(co - lo) * 4u;
-  return ldb(b, a.relb + (bounce ? dop : ds));
+  const uint32_t pb = (uint32_t)g.pb, D = (uint32_t)pull_span(g);
+  const uint32_t delta = (uint32_t)((ez * g.ny + ey) * (g.nx + (int)g.pa) + ex);
+  const uint32_t ds = ((uint32_t)k * pb - delta + D) * 4u;
+  const uint32_t dop = ((uint32_t)kopp * pb + D) * 4u;
+  return ldb(lo, relb + (bounce ? dop : ds));
+}
+// The pull base of a population pointer for a wave.
+__device__ __forceinline__ const float *pull_base(const float *p, const RowAddr &a, const Geom &g) {
+  return p + (a.base - pull_span(g));
 }
 
 // Pull stream of the 27 populations with halfway bounce-back, given the source
@@ -527,9 +532,10 @@ __device__ __forceinline__ void pull27(const Geom &g, int64_t i, const RowAddr &
   static_assert(KIND == ST_F32, "16-bit codecs pull in deviation form");
   using C = Codec<KIND>;
   float raw[Q27];
+  const float *lo = pull_base(f2, ra, g);
 #pragma unroll
   for (int k = 0; k < Q27; k++)
-    raw[k] = pull_row(f2, ra, g, k, opp27(k), EX27[k], EY27[k], EZ27[k], (nonfluid >> k) & 1u);
+    raw[k] = pull_row(lo, ra.relb, g, k, opp27(k), EX27[k], EY27[k], EZ27[k], (nonfluid >> k) & 1u);
   links27<typename C::R>(g, i, raw, solid, usx, usy, usz, rhoPhaseOld, exchange, x, y, z, cx, cy,
                          cz, mom, fp, pSumF, mx, my, mz);
 }
@@ -604,9 +610,10 @@ __device__ __forceinline__ void pull7(const Geom &g, const RowAddr &ra,
                                       const typename Codec<KIND>::T *__restrict__ h2,
                                       uint32_t nonfluid, float hp[Q7]) {
   static_assert(KIND == ST_F32, "16-bit codecs pull in deviation form");
+  const float *lo = pull_base(h2, ra, g);
 #pragma unroll
   for (int k = 0; k < Q7; k++)
-    hp[k] = pull_row(h2, ra, g, k, opp7(k), EX7[k], EY7[k], EZ7[k], (nonfluid >> k) & 1u);
+    hp[k] = pull_row(lo, ra.relb, g, k, opp7(k), EX7[k], EY7[k], EZ7[k], (nonfluid >> k) & 1u);
 }
 
 // Sum of a double over the 64 lanes of a wave by a fixed butterfly; every lane
@@ -669,6 +676,7 @@ struct Scene {
   int cgx, cgy;             // full-grid blocks of 64x4 cells: cgx * cgy * nz
   int band_blocks;          // blocks of 256 cells covering the largest band box
   int band_tile_cols;       // band-row tiles per plane a band-tile launch covers
+  int band_tile_planes;     // planes it covers (its blocks: cols * planes)
 };
 
 // Buffer of parity q (0 or 1) of a pair, as codec elements.  Both pointers are
@@ -952,12 +960,13 @@ __global__ __launch_bounds__(256) void k1_phase_stream(const Scene sc, int q) {
 // update reads their momentum): block b takes plane box[4] + b / M and, in it,
 // the (b % M)-th of the plane's band-row tiles not already counted for the
 // plane before (a tile can span two planes).  M = band_tile_cols; the host
-// sizes the launch for the largest box.  A plane with more band-row tiles than
-// M, or a box with more planes than the launch covers, sets kErrBandTiles.
+// sizes the launch (band_tile_planes planes of M blocks) for the largest box.
+// A plane with more band-row tiles than M, or a box with more planes than the
+// launch covers, sets kErrBandTiles.
 __device__ __forceinline__ bool band_tile(const Scene &sc, const int box[6], int b, int &tile) {
   const int M = sc.band_tile_cols;
   if (box[5] < box[4]) return false;
-  if (b == 0 && (box[5] - box[4] + 1) * M > (int)gridDim.x) atomicOr(&sc.ctrl->err, kErrBandTiles);
+  if (b == 0 && box[5] - box[4] + 1 > sc.band_tile_planes) atomicOr(&sc.ctrl->err, kErrBandTiles);
   const int z = box[4] + b / M;
   if (z > box[5]) return false;
   const TileMap &tm = sc.tm;
@@ -1266,13 +1275,19 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
       T *fob = fo + ra.base, *hob = ho + ra.base;
       finish_cell<COLLIDE, KIND>(
           sc, i, near, phi_next, gxo, gyo, gzo, lapo, fp, hp, pSumF, mx, my, mz, use_bf, phi_next,
-          [&](int k, float val, float base) { C::sto(fob + k * g.pb, ra.relb, val, base); },
-          [&](int k, float val, float base) { C::sto(hob + k * g.pb, ra.relb, val, base); });
+          [&](int k, float val, float base) {
+            C::sto(fob, ra.relb + (uint32_t)k * (uint32_t)g.pb * 4u, val, base);
+          },
+          [&](int k, float val, float base) {
+            C::sto(hob, ra.relb + (uint32_t)k * (uint32_t)g.pb * 4u, val, base);
+          });
     }
   }
   if (exchange) tile_momentum(mom, sc.partials, tile);
 }
-// Held to 128 VGPRs: four waves per SIMD.
+// Held to 128 VGPRs: four waves per SIMD; the band-row kernels and the
+// one-scene kernel for every tile (the last step of a call) to 168 (three
+// waves), where they fit without spilling.
 template <bool COLLIDE, int KIND>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void km_stream_collide(
     const Scene *__restrict__ scenes, const int2 *__restrict__ tiles, int q, int first) {
@@ -1280,18 +1295,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   stream_collide_body<COLLIDE, KIND>(scenes[t.x], t.y, q, first);
 }
 template <bool COLLIDE, int KIND>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k1_stream_collide(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k1_stream_collide(
     const Scene sc, int q, int first) {
   stream_collide_body<COLLIDE, KIND>(sc, blockIdx.x, q, first);
 }
-// The same over the bulk tiles and over the band-row tiles (one scene).
+// The same over the bulk tiles and over the band-row tiles, one scene (k1) or
+// a scene table (km: bulk over the tile table, band over each scene's band-tile
+// launch, found by its prefix table off).
+template <bool COLLIDE, int KIND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void km_stream_collide_bulk(
+    const Scene *__restrict__ scenes, const int2 *__restrict__ tiles, int q, int first) {
+  const int2 t = tiles[blockIdx.x];
+  stream_collide_body<COLLIDE, KIND, kTilesBulk>(scenes[t.x], t.y, q, first);
+}
+template <bool COLLIDE, int KIND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void km_stream_collide_band(
+    const Scene *__restrict__ scenes, const int *__restrict__ off, int nscenes, int q, int first) {
+  const int si = find_scene(off, nscenes, blockIdx.x);
+  stream_collide_body<COLLIDE, KIND, kTilesBand>(scenes[si], blockIdx.x - off[si], q, first);
+}
 template <bool COLLIDE, int KIND>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k1_stream_collide_bulk(
     const Scene sc, int q, int first) {
   stream_collide_body<COLLIDE, KIND, kTilesBulk>(sc, blockIdx.x, q, first);
 }
 template <bool COLLIDE, int KIND>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k1_stream_collide_band(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k1_stream_collide_band(
     const Scene sc, int q, int first) {
   stream_collide_body<COLLIDE, KIND, kTilesBand>(sc, blockIdx.x, q, first);
 }
@@ -1397,10 +1426,12 @@ __device__ __forceinline__ void put_half(uint32_t &w, int j, float v) {
   else C::template put<false>(w, v);
 }
 
-// red: 6 x 256 doubles of workgroup memory for the tile's link momentum.
-template <bool COLLIDE, int KIND, int PC, bool NTL>
-__device__ __forceinline__ void stream_collide_pair_body(const Scene &sc, int tile, int q,
-                                                         int first, double (*red)[256]) {
+// red: 6 x 256 doubles of workgroup memory for the tile's link momentum (not
+// used by bulk tiles).  MODE as stream_collide_body: bulk tiles carry no tag
+// reads and no momentum exchange.
+template <bool COLLIDE, int KIND, int PC, bool NTL, int MODE>
+__device__ __forceinline__ void stream_collide_pair_body(const Scene &sc, int tile, const int box[6],
+                                                         int q, int first, double (*red)[256]) {
   using C = Codec<KIND>;
   using R = typename C::R;
   static_assert(KIND != ST_F32, "16-bit codecs only");
@@ -1421,10 +1452,8 @@ __device__ __forceinline__ void stream_collide_pair_body(const Scene &sc, int ti
   float *__restrict__ phi_next = par(sc.phi, q ^ 1);
   const uint8_t *__restrict__ tag = sc.tag;
   const Ctrl *__restrict__ ctrl = sc.ctrl;
-  int box[6];
-#pragma unroll
-  for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
-  const bool exchange = sc.has_solid != 0 && tile_in_box_rows(sc.tm, box, tile);
+  const bool exchange = MODE != kTilesBulk && sc.has_solid != 0 &&
+                        (MODE == kTilesBand || tile_in_box_rows(sc.tm, box, tile));
   const int use_bf = first && sc.use_bf;
   double cx = 0, cy = 0, cz = 0;
   if (exchange) {
@@ -1441,7 +1470,7 @@ __device__ __forceinline__ void stream_collide_pair_body(const Scene &sc, int ti
   for (int J = 0; J < 2; J++) {
     const int64_t i = i0 + J;
     cell_xyz(g.nx, g.nxny, i, xs[J], ys[J], zs[J]);
-    nr[J] = in_box(box, xs[J], ys[J], zs[J]);
+    nr[J] = MODE != kTilesBulk && in_box(box, xs[J], ys[J], zs[J]);
     fl[J] = i < g.n && interior(g, xs[J], ys[J], zs[J]) && (!nr[J] || tag[i] == TAG_FLUID);
     nf[J] = so[J] = 0;
     if (fl[J]) {
@@ -1581,29 +1610,68 @@ __device__ __forceinline__ void stream_collide_pair_body(const Scene &sc, int ti
   }
 }
 
-// Workgroups of 64 x 2 threads, one tile each, held to at most 128 VGPRs so
-// that four waves share a SIMD (measured on MI355X: 0.68 against 0.72 ms at 3
-// waves for the 256^3 scene in FP16).
+// Workgroups of 64 x 2 threads, one tile each.  The bulk kernels are held to at
+// most 128 VGPRs so that four waves share a SIMD (measured on MI355X: 0.68
+// against 0.72 ms at 3 waves for the 256^3 scene in FP16, one launch for all
+// tiles then); the kernels that carry the band's paths (band-row tiles, and
+// every tile of one launch) to 168, three waves, where they fit without
+// spilling.  lb: the block's tile of a tile launch, or its index in a
+// band-tile launch.
 constexpr int kPairWaves = 2;
-template <bool COLLIDE, int KIND, bool NTL>
+template <bool COLLIDE, int KIND, bool NTL, int MODE = kTilesAll>
 __device__ __forceinline__ void stream_collide_pair(const Scene &sc, int lb, int q, int first) {
+  int box[6];
+#pragma unroll
+  for (int j = 0; j < 6; j++) box[j] = sc.ctrl->box_fused[j];
   int tile;
-  if (!map_tile(sc.tm.T, sc.tm.T8, lb, tile)) return;
-  __shared__ double red[6][256];
+  if constexpr (MODE == kTilesBand) {
+    if (!band_tile(sc, box, lb, tile)) return;
+  } else {
+    if (!map_tile(sc.tm.T, sc.tm.T8, lb, tile)) return;
+  }
+  if constexpr (MODE == kTilesBulk) {
+    if (tile_in_box_rows(sc.tm, box, tile)) return;
+  }
+  __shared__ double red[MODE == kTilesBulk ? 1 : 6][256];
   const int pc = parity_class(sc.g.nx, sc.g.ny);
-  if (pc == 0) stream_collide_pair_body<COLLIDE, KIND, 0, NTL>(sc, tile, q, first, red);
-  else if (pc == 1) stream_collide_pair_body<COLLIDE, KIND, 1, NTL>(sc, tile, q, first, red);
-  else stream_collide_pair_body<COLLIDE, KIND, 2, NTL>(sc, tile, q, first, red);
+  if (pc == 0) stream_collide_pair_body<COLLIDE, KIND, 0, NTL, MODE>(sc, tile, box, q, first, red);
+  else if (pc == 1) stream_collide_pair_body<COLLIDE, KIND, 1, NTL, MODE>(sc, tile, box, q, first, red);
+  else stream_collide_pair_body<COLLIDE, KIND, 2, NTL, MODE>(sc, tile, box, q, first, red);
 }
 template <bool COLLIDE, int KIND>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void km_stream_collide2(
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void km_stream_collide2(
     const Scene *__restrict__ scenes, const int2 *__restrict__ tiles, int q, int first) {
   const int2 t = tiles[blockIdx.x];
   stream_collide_pair<COLLIDE, KIND, false>(scenes[t.x], t.y, q, first);
 }
 template <bool COLLIDE, int KIND>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k1_stream_collide2(const Scene sc, int q, int first) {
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k1_stream_collide2(const Scene sc, int q, int first) {
   stream_collide_pair<COLLIDE, KIND, true>(sc, blockIdx.x, q, first);
+}
+// Bulk tiles and band-row tiles in separate launches, one scene (k1) or a
+// scene table (km: bulk over the tile table, band over each scene's band-tile
+// launch, found by its prefix table off).
+template <bool COLLIDE, int KIND>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k1_stream_collide2_bulk(
+    const Scene sc, int q, int first) {
+  stream_collide_pair<COLLIDE, KIND, true, kTilesBulk>(sc, blockIdx.x, q, first);
+}
+template <bool COLLIDE, int KIND>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k1_stream_collide2_band(
+    const Scene sc, int q, int first) {
+  stream_collide_pair<COLLIDE, KIND, true, kTilesBand>(sc, blockIdx.x, q, first);
+}
+template <bool COLLIDE, int KIND>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void km_stream_collide2_bulk(
+    const Scene *__restrict__ scenes, const int2 *__restrict__ tiles, int q, int first) {
+  const int2 t = tiles[blockIdx.x];
+  stream_collide_pair<COLLIDE, KIND, false, kTilesBulk>(scenes[t.x], t.y, q, first);
+}
+template <bool COLLIDE, int KIND>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void km_stream_collide2_band(
+    const Scene *__restrict__ scenes, const int *__restrict__ off, int nscenes, int q, int first) {
+  const int si = find_scene(off, nscenes, blockIdx.x);
+  stream_collide_pair<COLLIDE, KIND, false, kTilesBand>(scenes[si], blockIdx.x - off[si], q, first);
 }
 
 // phase_stream_body for two cells per lane (16-bit codecs), with the pair

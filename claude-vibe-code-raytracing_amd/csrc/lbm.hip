@@ -409,7 +409,7 @@ extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
   s->g.nxny = nx * ny;
   s->g.n = n;
   s->g.S = (n + 63) / 64 * 64;
-  set_layout(s->g, nx >= kRowLayoutMinNx ? LAYOUT_ROWS : LAYOUT_PLANES);
+  choose_layout(s->g);
   hipGetDevice(&s->device);
   scratch_setup(s->device);
   // src/sim.js:88-95 defaults.
@@ -1200,6 +1200,7 @@ __global__ __launch_bounds__(256) void k_pull_probe32(const float *__restrict__ 
   int x, y, z;
   cell_xyz(g.nx, g.nxny, i, x, y, z);
   const RowAddr ra = row_addr(g, i, z * g.ny + y);
+  const float *lo = pull_base(src, ra, g);
   float v[Q27 + Q7];
 #pragma unroll
   for (int k = 0; k < Q27 + Q7; k++) {
@@ -1210,10 +1211,10 @@ __global__ __launch_bounds__(256) void k_pull_probe32(const float *__restrict__ 
     // Direction k of the merged row block; the opposite of an h direction is
     // another h direction.
     const int kopp = k < Q27 ? opp27(k) : Q27 + opp7(kk);
-    v[k] = pull_row(src, ra, g, k, kopp, ex, ey, ez, !in);
+    v[k] = pull_row(lo, ra.relb, g, k, kopp, ex, ey, ez, !in);
   }
 #pragma unroll
-  for (int k = 0; k < Q27 + Q7; k++) stn(dst + ra.base + k * g.pb, ra.relb, v[k]);
+  for (int k = 0; k < Q27 + Q7; k++) stn(dst + ra.base, ra.relb + (uint32_t)k * (uint32_t)g.pb * 4u, v[k]);
 }
 __global__ __launch_bounds__(128) void k_pull_probe16(const uint16_t *__restrict__ src,
                                                       uint16_t *__restrict__ dst, Geom g,
@@ -1261,7 +1262,7 @@ extern "C" int lbm_probe_pull_bandwidth(int nx, int ny, int nz, int storage, int
   g.nxny = nx * ny;
   g.n = (int64_t)g.nxny * nz;
   g.S = (g.n + 63) / 64 * 64;
-  set_layout(g, nx >= kRowLayoutMinNx ? LAYOUT_ROWS : LAYOUT_PLANES);  // as lbm_create
+  choose_layout(g);  // as lbm_create
   const int elem = storage == LBM_STORAGE_F32 ? 4 : 2;
   const int64_t bytes = (int64_t)(Q27 + Q7) * g.S * elem;
   if (elem == 4 ? !f32_fused_fits(g) : bytes >= ((int64_t)1 << 32)) return LBM_ERR_ARG;
