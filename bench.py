@@ -129,6 +129,7 @@ def bench_batch(args, copy_gbps):
     """Many scenes stepped by the same launches (lbm_batch): aggregate MLUPS."""
     scenes = scenegen.config2(args.scenes, seed=args.seed + 1)
     batch = lbm.Batch()
+    batch.set_stream_split(args.batch_split)
     cells = 0
     keep = []
     for (nx, ny, nz), params, yw, sph in scenes:
@@ -381,6 +382,9 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=160.0, help="serve: GiB of scene state")
     ap.add_argument("--max-active", type=int, default=48, help="serve: active scene cap")
     ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--batch-split", type=int, default=-1, choices=[-1, 0, 1],
+                    help="batch: bulk/band-row split of the stream kernel (-1 by storage; "
+                         "the autotuner may change it)")
     ap.add_argument("--tune-steps", type=int, default=10,
                     help="single and batch: autotuner trial steps per candidate (0: no "
                          "autotuning)")

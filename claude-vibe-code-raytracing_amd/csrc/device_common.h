@@ -22,10 +22,11 @@ namespace lbm {
 //    of three neighbouring row blocks instead of 34 planes a whole grid apart:
 //    the 34-direction pull-and-store pattern runs at 5.6 TB/s against 5.25 for
 //    planes at 256^3 on MI355X (5.6 against 5.1 at 384^3);
-//  * planes (pa = 0, pb = S): direction k of every cell, then the next.  Narrow
-//    grids run faster so: a wave's 64 cells then span several rows, which row
-//    blocks scatter (64 scenes of 32..96 cells per axis stepped together: 8.96 k
-//    MLUPS in row blocks against 10.5 k in planes).
+//  * planes (pa = 0, pb = S): direction k of every cell, then the next.
+//    Narrower grids run faster so (64 scenes of 32..96 cells per axis stepped
+//    together: 10.5 k MLUPS in planes against 8.96 k in row blocks; served
+//    scenes of 32..192 cells per axis: 11.4 k against 10.5 k MLUPS with row
+//    blocks from nx >= 256 against nx >= 128).
 // lbm_create chooses row blocks when nx >= kRowLayoutMinNx (or when planes
 // would not fit the fused kernels' 32-bit offsets, f32_fused_fits).  Either way
 // a direction's source element is a uniform offset from the cell's own.
@@ -36,7 +37,7 @@ struct Geom {
 };
 
 constexpr int kRowQ = 27 + 7;
-constexpr int kRowLayoutMinNx = 128;
+constexpr int kRowLayoutMinNx = 256;
 enum : int { LAYOUT_PLANES = 0, LAYOUT_ROWS = 1 };
 inline void set_layout(Geom &g, int layout) {
   g.pa = layout == LAYOUT_ROWS ? (int64_t)(kRowQ - 1) * g.nx : 0;
@@ -121,7 +122,7 @@ inline bool f32_fused_fits(const Geom &g) {
   const int64_t span = 64 + 3 * (g.nx + g.pa) + (kRowQ - 1) * g.pb + 2 * pull_span(g);
   return span * 4 < ((int64_t)1 << 31);
 }
-// The layout lbm_create gives a grid (g.S set): row blocks from nx >= 128, or
+// The layout lbm_create gives a grid (g.S set): row blocks from nx >= 256, or
 // when planes would not fit the fused kernels' offsets.
 inline void choose_layout(Geom &g) {
   set_layout(g, LAYOUT_PLANES);

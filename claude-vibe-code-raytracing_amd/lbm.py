@@ -136,6 +136,7 @@ def lib():
         "lbm_step_noalloc": (C.c_int, [P, SP, C.c_int]),
         "lbm_step_live": (C.c_int, [P, SP, C.c_int]),
         "lbm_set_stream_split": (C.c_int, [P, C.c_int]),
+        "lbm_batch_set_stream_split": (C.c_int, [P, C.c_int]),
         "lbm_field_count": (C.c_int64, [P, C.c_int]),
         "lbm_download": (C.c_int, [P, C.c_int, P]),
         "lbm_upload": (C.c_int, [P, C.c_int, P]),
@@ -533,6 +534,12 @@ class Batch:
 
     def set_graph_steps(self, steps):
         self._ck(self.L.lbm_batch_set_graph_steps(self.h, int(steps)), "lbm_batch_set_graph_steps")
+
+    def set_stream_split(self, on):
+        """lbm_batch_set_stream_split: 1 split, 0 one launch, -1 by storage."""
+        rc = self.L.lbm_batch_set_stream_split(self.h, int(on))
+        if rc != 0:
+            raise LbmError(f"lbm_batch_set_stream_split = {rc}")
 
     def set_stream_lanes(self, cells_per_lane):
         self._ck(self.L.lbm_batch_set_stream_lanes(self.h, int(cells_per_lane)),

@@ -261,13 +261,17 @@ int lbm_batch_kernel_time(lbm_batch *batch, const char *name, double *total_ms, 
  * for any members whose launch grids match the captured ones. */
 int lbm_batch_set_graph_steps(lbm_batch *batch, int steps);
 int lbm_batch_graph_counts(const lbm_batch *batch, int64_t *captures, int64_t *replays);
+/* Bulk/band-row split of the batch's stream kernel, as lbm_set_stream_split:
+ * 1 split, 0 one launch, -1 (default) by storage kind (split for the 16-bit
+ * codecs, one launch for float32). */
+int lbm_batch_set_stream_split(lbm_batch *batch, int on);
 /* Cells per lane of the batch's 16-bit stream kernel, as lbm_set_stream_lanes. */
 int lbm_batch_set_stream_lanes(lbm_batch *batch, int cells_per_lane);
 
 /* Launch-shape autotuner.  No reference counterpart.  Times every candidate
  * launch configuration of the batch -- cells per lane of the 16-bit stream
- * kernel (when a member stores 16-bit populations), for one float32 simulation
- * (lbm_autotune) the stream kernel's bulk/band split (lbm_set_stream_split),
+ * kernel (when a member stores 16-bit populations), the stream kernel's
+ * bulk/band-row split (lbm_set_stream_split, lbm_batch_set_stream_split),
  * and steps per captured graph (0, 2, 8, 32, those below trial_steps - 1) --
  * over trial_steps real
  * coupled steps each, after an untimed lead-in of 2 steps, keeps the fastest
@@ -280,7 +284,7 @@ typedef struct {
   int graph_steps;     /* 0: kernels launched one by one */
   double ms_per_step;  /* of the chosen configuration */
   int candidates;      /* configurations timed */
-  int stream_split;    /* lbm_set_stream_split of the choice (float32, one simulation) */
+  int stream_split;    /* bulk/band-row split of the choice (1 split, 0 one launch) */
 } lbm_tune_result;
 int lbm_batch_autotune(lbm_batch *batch, int trial_steps, lbm_tune_result *out);
 /* The same for one simulation's lbm_step. */
@@ -439,11 +443,11 @@ int lbm_graph_counts(const lbm_sim *sim, int64_t *captures, int64_t *replays);
  * unspecified); float32 storage always runs one cell per lane. */
 int lbm_set_stream_lanes(lbm_sim *sim, int cells_per_lane);
 
-/* Launch shape of the float32 fused stream kernel for one simulation: 1
- * (default) launches the tiles away from the sphere by a kernel that carries
- * no tag reads and no momentum exchange, and the tiles of the sphere's band
- * rows by the full kernel; 0 launches every tile with the full kernel.  No
- * reference counterpart; results are the same bit for bit. */
+/* Launch shape of the fused stream kernel for one simulation: 1 (default)
+ * launches the tiles away from the sphere by a kernel that carries no tag
+ * reads and no momentum exchange, and the tiles of the sphere's band rows by
+ * the full kernel; 0 launches every tile with the full kernel.  No reference
+ * counterpart; results are the same bit for bit. */
 int lbm_set_stream_split(lbm_sim *sim, int on);
 
 /* Measured device bandwidth of a plain copy: copies `bytes` (rounded down to 16)

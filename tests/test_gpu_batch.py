@@ -51,10 +51,14 @@ def assert_same(a, c, what):
     assert a[1] == c[1], what
 
 
+@pytest.mark.parametrize("split", [-1, 0, 1])
 @pytest.mark.parametrize("subset", [[0, 1, 2, 3], [0, 1, 2, 3, 4]])
-def test_batch_members_equal_alone(subset):
+def test_batch_members_equal_alone(subset, split):
+    """Whatever the batch's bulk/band-row split of its stream launches (one
+    launch, split, or chosen by storage kind), each member ends where it ends
+    alone (with the single simulation's default split)."""
     steps = 24
-    announce("batch vs alone", subset)
+    announce("batch vs alone", subset, "split", split)
     alone = []
     for j in subset:
         sim, b = make(SCENES[j])
@@ -62,6 +66,7 @@ def test_batch_members_equal_alone(subset):
         alone.append(snapshot(sim, b))
         sim.close()
     batch = lbm.Batch()
+    batch.set_stream_split(split)
     sims = []
     for j in subset:
         sim, b = make(SCENES[j])
