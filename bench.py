@@ -11,13 +11,16 @@ continuing run (lbm_step_live: no per-call prologue or closing stream-only
 step inside the timed region, every step complete; `reference_state_call`
 gives the same steps as one call from and to the reference's state).  Launch
 shapes come from the autotuner; no timing events in the timed region.  The
-roofline object is for the dominant kernel (stream + macro + next collide, the
-bulk and band-row launches of a step together), timed with HIP events on the
-library's stream over a further call; its algorithmic traffic is 281 B per
-cell in float32 (27+7 populations read and written, phi(t), phi(t+1) and the
-tag read; bulk cells store rho and u only at the last step of a call) and
-145 B with 16-bit populations.  The whole-step fraction uses 297 B (161 B) per
-cell (the same plus the phi write of the phase pass and the rho/u writes).
+roofline object is for the dominant kernel (stream + macro + next collide),
+timed with HIP events on the library's stream over a further call.  For
+float32 at 256^3 that is the row march with the band box's launch beside it
+(k1_march computes phi(t+1) itself: 280 B per cell, the 27+7 populations read
+and written, phi(t) read and phi(t+1) written); otherwise the bulk and
+band-row launches of the stream kernel (281 B per cell in float32: the
+populations, phi(t), phi(t+1) and the tag read; 145 B with 16-bit
+populations), after a phase pass of their own.  Bulk cells store rho and u
+only at the last step of a call.  The whole-step fraction uses 297 B (161 B)
+per cell (populations, phi read and written, rho/u written, the tag).
 `traffic` is the measured HBM bytes per launch from the committed counter
 profiles (rocprofv3 FETCH_SIZE/WRITE_SIZE calibrated on a copy kernel), for
 the 256^3 scene only.
@@ -53,6 +56,11 @@ HBM_PEAK_GBS = 8000.0
 # phi(t+1); step: 34 populations read and written + phi read and written +
 # rho/u written + tag.
 KERNEL_BYTES_PER_LU = {"f32": 281, "f16": 145, "bf16": 145}
+# The float32 row march (lbm_set_stream_split 2) computes phi(t+1) itself: it
+# reads the 27 + 7 populations and phi(t), writes the 27 + 7 populations and
+# phi(t+1) (the tag only inside the band box).
+MARCH_BYTES_PER_LU = 280
+MARCH_KERNEL = "k1_march + k1_stream_collide_box"
 STEP_BYTES_PER_LU = {"f32": 297, "f16": 161, "bf16": 161}
 # Populations alone (27 + 7 planes read and written): the pattern probe's bytes.
 PATTERN_BYTES_PER_LU = {"f32": 272, "f16": 136, "bf16": 136}
@@ -328,7 +336,13 @@ def single_leg(args, storage, copy_gbps):
     mlups = cells * args.steps / dt / 1e6
     ms_sc, cnt_sc = kt["stream_collide"]
     avg_s = ms_sc / max(cnt_sc, 1) / 1e3
-    kb = KERNEL_BYTES_PER_LU[storage]
+    # The row march runs for float32 rows of at most 256 cells when the
+    # autotuner (or the default) chose it; its launch pair replaces the phase
+    # pass and the stream kernel.
+    split = tune["stream_split"] if tune else 2
+    march = storage == "f32" and split == 2 and n <= 256
+    kb = MARCH_BYTES_PER_LU if march else KERNEL_BYTES_PER_LU[storage]
+    kname = MARCH_KERNEL if march else STREAM_KERNEL[storage]
     achieved = kb * cells / avg_s / 1e9 if cnt_sc else None
     phi = sim.get("phi")
     finite = bool(np.isfinite(phi).all())
@@ -339,7 +353,7 @@ def single_leg(args, storage, copy_gbps):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": measured_traffic(n, storage),
-                     "kernel": STREAM_KERNEL[storage], "bytes_per_lu": kb,
+                     "kernel": kname, "bytes_per_lu": kb,
                      "avg_ms": round(avg_s * 1e3, 4),
                      "timed_over": f"{args.steps} steps with HIP events after the timed call "
                                    f"({dt_ev / args.steps * 1e3:.4f} ms/step with events)"},

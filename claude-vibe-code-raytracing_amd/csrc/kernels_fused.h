@@ -994,7 +994,8 @@ __device__ __forceinline__ bool band_tile(const Scene &sc, const int box[6], int
 // and momentum mx, my, mz; hp) and the gradients of phi(t) at i.  band: the cell
 // lies in the band box (its collide(t+1) runs after the band update).  The 27 +
 // 7 populations to store are handed to stf(k, value, baseline) and
-// sth(k, value, baseline).
+// sth(k, value, baseline).  phi1: the stencil of phi(t+1) around the cell when
+// the caller holds it (the row march), else it is loaded from phi_next.
 template <bool COLLIDE, int KIND, typename StF, typename StH>
 __device__ __forceinline__ void finish_cell(const Scene &sc, int64_t i, bool band,
                                             const float *__restrict__ phi_next, float gxo,
@@ -1003,7 +1004,8 @@ __device__ __forceinline__ void finish_cell(const Scene &sc, int64_t i, bool ban
                                             typename Codec<KIND>::R mx,
                                             typename Codec<KIND>::R my,
                                             typename Codec<KIND>::R mz, int use_bf,
-                                            float *__restrict__ phi_out, StF &&stf, StH &&sth) {
+                                            float *__restrict__ phi_out, StF &&stf, StH &&sth,
+                                            const Phi7 *phi1 = nullptr) {
   using C = Codec<KIND>;
   using R = typename C::R;
   const Geom &g = sc.g;
@@ -1058,7 +1060,8 @@ __device__ __forceinline__ void finish_cell(const Scene &sc, int64_t i, bool ban
       for (int k = 0; k < Q7; k++) hp[k] = C::rt(hp[k], w7(k) * phif);
     }
     float gx1, gy1, gz1, lap1;
-    phi_grad<R>(g, phi_next, i, (R)phif, gx1, gy1, gz1, lap1);
+    if (phi1) phi_grad7<R>(*phi1, (R)phif, gx1, gy1, gz1, lap1);
+    else phi_grad<R>(g, phi_next, i, (R)phif, gx1, gy1, gz1, lap1);
     float hc[Q7];
     collide_phase_cell<R>(d, (R)phif, (R)uxf, (R)uyf, (R)uzf, (R)gx1, (R)gy1, (R)gz1, hp, hc);
 #pragma unroll
@@ -1155,8 +1158,10 @@ __device__ __forceinline__ void finish_cell_dev(const Scene &sc, int64_t i, bool
 // tiles, whose cells all lie outside the band box (no tag is read, no link
 // momentum arises: the kernel carries neither path); kTilesBand only the
 // band-row tiles (band_tile).  Bulk and band launches together cover every
-// tile once.
-constexpr int kTilesAll = 0, kTilesBulk = 1, kTilesBand = 2;
+// tile once.  kTilesBox: the band-row tiles' cells inside the band box only
+// (the row march updates every other cell); lanes outside it add nothing to
+// the tile's momentum, as in kTilesBand, so both give the same partials.
+constexpr int kTilesAll = 0, kTilesBulk = 1, kTilesBand = 2, kTilesBox = 3;
 template <bool COLLIDE, int KIND, int MODE = kTilesAll>
 __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int q, int first) {
   using C = Codec<KIND>;
@@ -1169,7 +1174,7 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
 #pragma unroll
   for (int j = 0; j < 6; j++) box[j] = ctrl->box_fused[j];
   int tile, x, y, z;
-  if constexpr (MODE == kTilesBand) {
+  if constexpr (MODE == kTilesBand || MODE == kTilesBox) {
     if (!band_tile(sc, box, lb, tile)) return;
   } else {
     if (!map_tile(sc.tm.T, sc.tm.T8, lb, tile)) return;
@@ -1177,7 +1182,8 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
   // Link momentum can only arise next to the sphere, inside the band box: the
   // tiles holding the box's rows store their sums (zero or not), and the body
   // update reads exactly those.
-  const bool box_rows = MODE == kTilesBand || (MODE == kTilesAll && tile_in_box_rows(sc.tm, box, tile));
+  const bool box_rows = MODE == kTilesBand || MODE == kTilesBox ||
+                        (MODE == kTilesAll && tile_in_box_rows(sc.tm, box, tile));
   if constexpr (MODE == kTilesBulk) {
     if (tile_in_box_rows(sc.tm, box, tile)) return;
   }
@@ -1206,7 +1212,8 @@ __device__ __forceinline__ void stream_collide_body(const Scene &sc, int lb, int
     // only non-fluid cells are the walls on the grid faces.
     const bool near = MODE != kTilesBulk && in_box(box, x, y, z);
     const uint32_t bo4 = (uint32_t)i * 4u;
-    const bool fluid = interior(g, x, y, z) && (!near || ldb(tag, (uint32_t)i) == TAG_FLUID);
+    const bool fluid = (MODE != kTilesBox || near) && interior(g, x, y, z) &&
+                       (!near || ldb(tag, (uint32_t)i) == TAG_FLUID);
     if (!fluid) {
       if (!COLLIDE) stb(phi_next, bo4, ldb(phi, bo4));
     } else if constexpr (KIND != ST_F32) {
@@ -1323,6 +1330,12 @@ template <bool COLLIDE, int KIND>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k1_stream_collide_band(
     const Scene sc, int q, int first) {
   stream_collide_body<COLLIDE, KIND, kTilesBand>(sc, blockIdx.x, q, first);
+}
+// The band box's cells of the band-row tiles (beside the row march).
+template <int KIND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k1_stream_collide_box(
+    const Scene sc, int q, int first) {
+  stream_collide_body<true, KIND, kTilesBox>(sc, blockIdx.x, q, first);
 }
 // ---------------------------------------------------------------- stream + collide, two cells per lane
 
@@ -1624,7 +1637,7 @@ __device__ __forceinline__ void stream_collide_pair(const Scene &sc, int lb, int
 #pragma unroll
   for (int j = 0; j < 6; j++) box[j] = sc.ctrl->box_fused[j];
   int tile;
-  if constexpr (MODE == kTilesBand) {
+  if constexpr (MODE == kTilesBand || MODE == kTilesBox) {
     if (!band_tile(sc, box, lb, tile)) return;
   } else {
     if (!map_tile(sc.tm.T, sc.tm.T8, lb, tile)) return;

@@ -19,6 +19,7 @@
 #include "device_common.h"
 #include "kernels_pass.h"
 #include "kernels_fused.h"
+#include "kernels_march.h"
 
 using namespace lbm;
 
@@ -132,9 +133,11 @@ struct lbm_sim {
   Timer timer;
   GraphCache graphs;  // steady steps of the fused call (lbm_step)
   int lanes = 2;      // cells per lane of the 16-bit stream kernel (1 or 2)
-  // Float32 fused steps launch the bulk tiles and the band-row tiles of the
+  // Float32 fused steps run the row march beside the band box's cells (2,
+  // where rows fit it), launch the bulk tiles and the band-row tiles of the
   // stream kernel separately (1) or as one launch (0).
-  int stream_split = 1;
+  int stream_split = 2;
+  int march_nseg = 0;  // plane segments of the row march (0: chosen per grid)
   std::string err;
 };
 
@@ -1152,10 +1155,19 @@ extern "C" int lbm_set_graph_steps(lbm_sim *s, int steps) {
   return LBM_OK;
 }
 
-extern "C" int lbm_set_stream_split(lbm_sim *s, int on) {
+extern "C" int lbm_set_stream_split(lbm_sim *s, int mode) {
   CHECK_SIM(s);
-  if (on != 0 && on != 1) return fail(s, LBM_ERR_ARG, "stream split must be 0 or 1");
-  s->stream_split = on;
+  if (mode < 0 || mode > 2) return fail(s, LBM_ERR_ARG, "stream split must be 0, 1 or 2");
+  s->stream_split = mode;
+  s->graphs.clear();
+  return LBM_OK;
+}
+
+extern "C" int lbm_set_march_segments(lbm_sim *s, int segments) {
+  CHECK_SIM(s);
+  if (segments < 0) return fail(s, LBM_ERR_ARG, "march segments < 0");
+  s->march_nseg = segments;
+  s->graphs.clear();
   return LBM_OK;
 }
 

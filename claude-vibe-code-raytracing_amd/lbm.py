@@ -136,6 +136,7 @@ def lib():
         "lbm_step_noalloc": (C.c_int, [P, SP, C.c_int]),
         "lbm_step_live": (C.c_int, [P, SP, C.c_int]),
         "lbm_set_stream_split": (C.c_int, [P, C.c_int]),
+        "lbm_set_march_segments": (C.c_int, [P, C.c_int]),
         "lbm_batch_set_stream_split": (C.c_int, [P, C.c_int]),
         "lbm_field_count": (C.c_int64, [P, C.c_int]),
         "lbm_download": (C.c_int, [P, C.c_int, P]),
@@ -458,8 +459,13 @@ class Sim:
         """Steps per captured launch graph of coupled(); 0 launches kernels one by one."""
         self._ck(self.L.lbm_set_graph_steps(self.h, int(steps)), "lbm_set_graph_steps")
 
-    def set_stream_split(self, on):
-        self._ck(self.L.lbm_set_stream_split(self.h, int(on)), "lbm_set_stream_split")
+    def set_stream_split(self, mode):
+        """lbm_set_stream_split: 1 bulk/band-row split, 0 one launch, 2 the row
+        march (float32, rows of at most 256 cells)."""
+        self._ck(self.L.lbm_set_stream_split(self.h, int(mode)), "lbm_set_stream_split")
+
+    def set_march_segments(self, segments):
+        self._ck(self.L.lbm_set_march_segments(self.h, int(segments)), "lbm_set_march_segments")
 
     def set_stream_lanes(self, cells_per_lane):
         """Cells per lane (1 or 2) of the 16-bit stream kernel."""

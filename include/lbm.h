@@ -443,12 +443,19 @@ int lbm_graph_counts(const lbm_sim *sim, int64_t *captures, int64_t *replays);
  * unspecified); float32 storage always runs one cell per lane. */
 int lbm_set_stream_lanes(lbm_sim *sim, int cells_per_lane);
 
-/* Launch shape of the fused stream kernel for one simulation: 1 (default)
- * launches the tiles away from the sphere by a kernel that carries no tag
- * reads and no momentum exchange, and the tiles of the sphere's band rows by
- * the full kernel; 0 launches every tile with the full kernel.  No reference
- * counterpart; results are the same bit for bit. */
-int lbm_set_stream_split(lbm_sim *sim, int on);
+/* Launch shape of the fused stream kernel for one simulation: 1 launches the
+ * tiles away from the sphere by a kernel that carries no tag reads and no
+ * momentum exchange, and the tiles of the sphere's band rows by the full
+ * kernel; 0 launches every tile with the full kernel; 2 (default; float32
+ * storage with rows of at most 256 cells, otherwise as 1) runs the row march:
+ * one launch computes phi(t+1) and streams and collides every cell outside the
+ * sphere's band box, row by row along z, beside a launch over the band box's
+ * cells.  No reference counterpart; results are the same bit for bit. */
+int lbm_set_stream_split(lbm_sim *sim, int mode);
+/* Plane segments of the row march (mode 2 above): each workgroup marches one
+ * row through nz / segments planes; 0 (default) chooses about 1024
+ * workgroups.  No reference counterpart; results do not depend on it. */
+int lbm_set_march_segments(lbm_sim *sim, int segments);
 
 /* Measured device bandwidth of a plain copy: copies `bytes` (rounded down to 16)
  * between two device buffers `reps` times with one float4 per thread and
