@@ -26,7 +26,8 @@ profiles (rocprofv3 FETCH_SIZE/WRITE_SIZE calibrated on a copy kernel), for
 the 256^3 scene only.
 
 The default line also carries the FP16-deviation form on the same scene
-(`f16`, with its ratio to float32) and the serving loop (config 4: Poisson
+(`f16`, with its ratio to float32), config 2 stepped as one batch in both
+forms (`batch`, with the ratio) and the serving loop (config 4: Poisson
 arrivals into the request loop) in both forms for --serve-seconds each
 (`serve`: served MLUPS, frame latency and time-to-first-frame percentiles).
 Other workloads: --workload batch (config 2) and --workload serve.
@@ -133,8 +134,10 @@ def cpu_baseline(n, seed, steps, parallel=False):
                       f"{cpu_model()}"}
 
 
-def bench_batch(args, copy_gbps):
-    """Many scenes stepped by the same launches (lbm_batch): aggregate MLUPS."""
+def batch_run(args, storage, steps):
+    """Config 2: the seeded scenes stepped together by the same launches
+    (lbm_batch) with populations stored as `storage`: warm-up, the launch-shape
+    autotuner, then `steps` timed steps; aggregate MLUPS and per-kernel times."""
     scenes = scenegen.config2(args.scenes, seed=args.seed + 1)
     batch = lbm.Batch()
     batch.set_stream_split(args.batch_split)
@@ -144,58 +147,78 @@ def bench_batch(args, copy_gbps):
         sim = lbm.Sim(nx, ny, nz)
         sim.set_params(**params)
         sim.init_waterline_map(yw)
-        sim.set_storage(args.storage)
+        sim.set_storage(storage)
         b = lbm.make_sphere(nx, ny, nz, sph["cx"], sph["cy"], sph["cz"], sph["r"], sph["density"])
         batch.add(sim, b)
         keep.append((sim, b))
         cells += nx * ny * nz
-    print(f"[bench] {len(scenes)} scenes, {cells} cells; warmup {args.warmup}", file=sys.stderr,
-          flush=True)
+    print(f"[bench] batch {storage}: {len(scenes)} scenes, {cells} cells; warmup {args.warmup}",
+          file=sys.stderr, flush=True)
     batch.step(args.warmup)
     # Launch shapes from the autotuner (its trial steps are ordinary steps).
     tune = batch.autotune(args.tune_steps) if args.tune_steps > 0 else None
     # The timed call launches as in production (steady steps replayed from a
     # captured graph); lbm_batch_step returns after its stream has drained.
     t0 = time.perf_counter()
-    batch.step(args.steps)
+    batch.step(steps)
     dt = time.perf_counter() - t0
     # Per-kernel times from a separate call with HIP events around every launch
     # (kernels launched one by one).
-    ksteps = min(args.steps, 100)
+    ksteps = min(steps, 100)
     batch.set_timing(True)
     batch.step(ksteps)
     kt = {k: batch.kernel_time(k) for k in ("stream_collide", "phase_stream", "stream_macro",
                                            "body", "band", "collide_all")}
     batch.set_timing(False)
-    mlups = cells * args.steps / dt / 1e6
+    mlups = cells * steps / dt / 1e6
     ms_sc, cnt_sc = kt["stream_collide"]
     avg_s = ms_sc / max(cnt_sc, 1) / 1e3
-    kb = KERNEL_BYTES_PER_LU[args.storage]
+    kb = KERNEL_BYTES_PER_LU[storage]
     achieved = kb * cells / avg_s / 1e9 if cnt_sc else None
     finite = all(bool(np.isfinite(s.get("phi")).all()) for s, _ in keep[:4])
-    out = {
-        "metric": "batched scenes aggregate MLUPS (config 2)",
-        "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": ARITH[args.storage], "data": "synthetic",
-        "config": {"workload": f"config 2: {len(scenes)} seeded scenes, axes 32..96, one sphere "
-                               f"each, {STORAGE_NAME[args.storage]} populations, one batch",
-                   "scenes": len(scenes), "cells": cells, "storage": args.storage,
-                   "arithmetic": ARITH[args.storage], "seed": args.seed + 1},
+    batch.close()
+    for sim, _ in keep:
+        sim.close()
+    return {
+        "storage": storage, "mlups": round(mlups, 2), "ms_per_step": round(dt / steps * 1e3, 4),
+        "steps": steps, "scenes": len(scenes), "cells": cells,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None, "kernel": STREAM_KERNEL[args.storage].replace("k1_", "km_")
+                     "traffic": None, "kernel": STREAM_KERNEL[storage].replace("k1_", "km_")
                      + " (all scenes per launch)",
                      "bytes_per_lu": kb, "avg_ms": round(avg_s * 1e3, 4),
                      "timed_over": f"{ksteps} steps with HIP events after the timed call"},
-        "whole_step_hbm_frac": round(STEP_BYTES_PER_LU[args.storage] * mlups * 1e6 /
+        "whole_step_hbm_frac": round(STEP_BYTES_PER_LU[storage] * mlups * 1e6 /
                                      (HBM_PEAK_GBS * 1e9), 4),
-        "copy_probe_gbps": round(copy_gbps, 1),
         "kernel_ms": {k: {"total": round(v[0], 3), "launches": v[1]} for k, v in kt.items()},
         "finite": finite,
         "autotune": tune,
+    }
+
+
+BATCH_WORKLOAD = "config 2: {n} seeded scenes, axes 32..96, one sphere each, one batch"
+
+
+def bench_batch(args, copy_gbps):
+    """The batched workload alone (--workload batch): one JSON line."""
+    r = batch_run(args, args.storage, args.steps)
+    out = {
+        "metric": "batched scenes aggregate MLUPS (config 2)",
+        "value": r["mlups"], "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": ARITH[args.storage], "data": "synthetic",
+        "config": {"workload": BATCH_WORKLOAD.format(n=r["scenes"])
+                   + f", {STORAGE_NAME[args.storage]} populations",
+                   "scenes": r["scenes"], "cells": r["cells"], "storage": args.storage,
+                   "arithmetic": ARITH[args.storage], "seed": args.seed + 1},
+        "roofline": r["roofline"],
+        "whole_step_hbm_frac": r["whole_step_hbm_frac"],
+        "copy_probe_gbps": round(copy_gbps, 1),
+        "kernel_ms": r["kernel_ms"],
+        "finite": r["finite"],
+        "autotune": r["autotune"],
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -402,9 +425,12 @@ def main():
     ap.add_argument("--tune-steps", type=int, default=10,
                     help="single and batch: autotuner trial steps per candidate (0: no "
                          "autotuning)")
-    ap.add_argument("--legs", default="f16,serve",
+    ap.add_argument("--legs", default="f16,batch,serve",
                     help="single workload: extra legs in the line (f16: the FP16-deviation form "
-                         "on the same scene; serve: the serving loop in both forms)")
+                         "on the same scene; batch: config 2 in both forms; serve: the serving "
+                         "loop in both forms)")
+    ap.add_argument("--batch-steps", type=int, default=200,
+                    help="single workload: timed steps of each batch leg")
     ap.add_argument("--serve-seconds", type=float, default=30.0,
                     help="single workload: seconds of each serving leg")
     ap.add_argument("--storage", choices=["f32", "f16", "bf16"], default="f32",
@@ -464,6 +490,13 @@ def main():
             f16.pop(k)
         f16["over_f32"] = round(f16["mlups"] / head["mlups"], 3)
         out["f16"] = f16
+    if "batch" in args.legs and args.batch_steps > 0:
+        batch = {}
+        for st in ("f32", "f16"):
+            batch[st] = batch_run(args, st, args.batch_steps)
+        batch["f16_over_f32"] = round(batch["f16"]["mlups"] / batch["f32"]["mlups"], 3)
+        batch["workload"] = BATCH_WORKLOAD.format(n=batch["f32"]["scenes"])
+        out["batch"] = batch
     if "serve" in args.legs and args.serve_seconds > 0:
         serve = {}
         for st in ("f32", "f16"):

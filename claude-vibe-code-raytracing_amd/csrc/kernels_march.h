@@ -27,10 +27,6 @@
 namespace lbm {
 
 constexpr int kMarchCells = 256;  // cells of a row a workgroup covers (one per lane)
-// Rows per workgroup of the launches: with two, each lane computes one halo
-// row's phi(t+1) per plane instead of two (256^3 on MI355X: march + band box
-// launches 1.075 against 1.085 ms per step with one row).
-constexpr int kMarchRows = 2;
 
 // LDS of a marching workgroup of RPW rows.  Plane z of an own row's phi(t+1)
 // sits in slot z & 3 of p, of a halo row and of the own rows' pulled h in slot

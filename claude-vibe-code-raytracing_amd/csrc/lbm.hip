@@ -137,6 +137,7 @@ struct lbm_sim {
   // where rows fit it), launch the bulk tiles and the band-row tiles of the
   // stream kernel separately (1) or as one launch (0).
   int stream_split = 2;
+  int march_rows = 2;  // rows per workgroup of the row march (1 or 2)
   int march_nseg = 0;  // plane segments of the row march (0: chosen per grid)
   std::string err;
 };
@@ -1163,9 +1164,11 @@ extern "C" int lbm_set_stream_split(lbm_sim *s, int mode) {
   return LBM_OK;
 }
 
-extern "C" int lbm_set_march_segments(lbm_sim *s, int segments) {
+extern "C" int lbm_set_march_shape(lbm_sim *s, int rows, int segments) {
   CHECK_SIM(s);
+  if (rows != 1 && rows != 2) return fail(s, LBM_ERR_ARG, "march rows must be 1 or 2");
   if (segments < 0) return fail(s, LBM_ERR_ARG, "march segments < 0");
+  s->march_rows = rows;
   s->march_nseg = segments;
   s->graphs.clear();
   return LBM_OK;

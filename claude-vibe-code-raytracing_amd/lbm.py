@@ -88,7 +88,8 @@ class ServerStats(C.Structure):
 class TuneResult(C.Structure):
     """lbm_tune_result: the launch configuration the autotuner chose."""
     _fields_ = [("cells_per_lane", C.c_int), ("graph_steps", C.c_int),
-                ("ms_per_step", C.c_double), ("candidates", C.c_int), ("stream_split", C.c_int)]
+                ("ms_per_step", C.c_double), ("candidates", C.c_int), ("stream_split", C.c_int),
+                ("march_rows", C.c_int), ("march_segments", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -136,7 +137,7 @@ def lib():
         "lbm_step_noalloc": (C.c_int, [P, SP, C.c_int]),
         "lbm_step_live": (C.c_int, [P, SP, C.c_int]),
         "lbm_set_stream_split": (C.c_int, [P, C.c_int]),
-        "lbm_set_march_segments": (C.c_int, [P, C.c_int]),
+        "lbm_set_march_shape": (C.c_int, [P, C.c_int, C.c_int]),
         "lbm_batch_set_stream_split": (C.c_int, [P, C.c_int]),
         "lbm_field_count": (C.c_int64, [P, C.c_int]),
         "lbm_download": (C.c_int, [P, C.c_int, P]),
@@ -464,8 +465,9 @@ class Sim:
         march (float32, rows of at most 256 cells)."""
         self._ck(self.L.lbm_set_stream_split(self.h, int(mode)), "lbm_set_stream_split")
 
-    def set_march_segments(self, segments):
-        self._ck(self.L.lbm_set_march_segments(self.h, int(segments)), "lbm_set_march_segments")
+    def set_march_shape(self, rows=2, segments=0):
+        self._ck(self.L.lbm_set_march_shape(self.h, int(rows), int(segments)),
+                 "lbm_set_march_shape")
 
     def set_stream_lanes(self, cells_per_lane):
         """Cells per lane (1 or 2) of the 16-bit stream kernel."""

@@ -272,8 +272,10 @@ int lbm_batch_set_stream_lanes(lbm_batch *batch, int cells_per_lane);
  * launch configuration of the batch -- cells per lane of the 16-bit stream
  * kernel (when a member stores 16-bit populations), the stream kernel's
  * bulk/band-row split (lbm_set_stream_split, lbm_batch_set_stream_split),
- * and steps per captured graph (0, 2, 8, 32, those below trial_steps - 1) --
- * over trial_steps real
+ * for one float32 simulation whose rows fit it the row march in three
+ * workgroup shapes (lbm_set_march_shape: two rows and the default segments,
+ * one row, two rows and twice the segments), and steps per captured graph
+ * (0, 2, 8, 32, those below trial_steps - 1) -- over trial_steps real
  * coupled steps each, after an untimed lead-in of 2 steps, keeps the fastest
  * and reports it.  Every candidate computes the same state, so the trials
  * simply advance the members: after the call each has advanced by
@@ -284,7 +286,9 @@ typedef struct {
   int graph_steps;     /* 0: kernels launched one by one */
   double ms_per_step;  /* of the chosen configuration */
   int candidates;      /* configurations timed */
-  int stream_split;    /* bulk/band-row split of the choice (1 split, 0 one launch) */
+  int stream_split;    /* stream launch shape of the choice (1 split, 0 one launch, 2 march) */
+  int march_rows;      /* rows per workgroup of the row march (1 or 2) */
+  int march_segments;  /* plane segments of the row march (0: chosen per grid) */
 } lbm_tune_result;
 int lbm_batch_autotune(lbm_batch *batch, int trial_steps, lbm_tune_result *out);
 /* The same for one simulation's lbm_step. */
@@ -452,10 +456,11 @@ int lbm_set_stream_lanes(lbm_sim *sim, int cells_per_lane);
  * sphere's band box, row by row along z, beside a launch over the band box's
  * cells.  No reference counterpart; results are the same bit for bit. */
 int lbm_set_stream_split(lbm_sim *sim, int mode);
-/* Plane segments of the row march (mode 2 above): each workgroup marches one
- * row through nz / segments planes; 0 (default) chooses about 1024
- * workgroups.  No reference counterpart; results do not depend on it. */
-int lbm_set_march_segments(lbm_sim *sim, int segments);
+/* Workgroup shape of the row march (mode 2 above): each workgroup marches
+ * `rows` rows (1 or 2; default 2) through nz / segments planes; segments 0
+ * (default) chooses about 4096 / rows waves in all.  No reference
+ * counterpart; results do not depend on it. */
+int lbm_set_march_shape(lbm_sim *sim, int rows, int segments);
 
 /* Measured device bandwidth of a plain copy: copies `bytes` (rounded down to 16)
  * between two device buffers `reps` times with one float4 per thread and

@@ -24,10 +24,10 @@ MARCH_SCENES = [
 CALLS = [5, 12, 1, 9]
 
 
-def run(spec, mode, segments=0):
+def run(spec, mode, segments=0, rows=2):
     sim, b = make(spec)
     sim.set_stream_split(mode)
-    sim.set_march_segments(segments)
+    sim.set_march_shape(rows, segments)
     for n in CALLS:
         sim.coupled(b, n)
     out = snapshot(sim, b)
@@ -35,19 +35,42 @@ def run(spec, mode, segments=0):
     return out
 
 
+@pytest.mark.parametrize("rows", [2, 1])
 @pytest.mark.parametrize("j", range(len(MARCH_SCENES)))
-def test_march_equals_split(j):
+def test_march_equals_split(j, rows):
     spec = MARCH_SCENES[j]
-    announce("row march vs split", spec)
+    announce("row march vs split", spec, "rows per workgroup", rows)
     ref = run(spec, 1)
-    assert_same(run(spec, 2), ref, ("march", spec))
+    assert_same(run(spec, 2, rows=rows), ref, ("march", spec, rows))
 
 
+@pytest.mark.parametrize("rows", [2, 1])
 @pytest.mark.parametrize("segments", [1, 3, 7])
-def test_march_segments_do_not_change_results(segments):
+def test_march_segments_do_not_change_results(segments, rows):
     spec = MARCH_SCENES[2]
-    announce("row march with", segments, "segments", spec)
-    assert_same(run(spec, 2, segments), run(spec, 1), ("segments", segments))
+    announce("row march with", segments, "segments", rows, "rows", spec)
+    assert_same(run(spec, 2, segments, rows), run(spec, 1), ("segments", segments, rows))
+
+
+def test_march_autotuned_shape():
+    """The autotuner times the march shapes among its candidates; whatever it
+    keeps, the state is that of plain calls of the same length."""
+    spec = MARCH_SCENES[1]
+    announce("autotuned march", spec)
+    sim, b = make(spec)
+    r = sim.autotune(b, 4)
+    assert r["march_rows"] in (1, 2) and r["march_segments"] >= 0, r
+    # 3 split modes: one launch, split, march x 3 shapes; graph steps 0 and 2
+    assert r["candidates"] == (1 + 1 + 3) * 2, r
+    total = r["candidates"] * (4 + 2)
+    sim.coupled(b, 7)
+    tuned = snapshot(sim, b)
+    sim.close()
+    sim, b = make(spec)
+    sim.coupled(b, total)
+    sim.coupled(b, 7)
+    assert_same(tuned, snapshot(sim, b), "autotuned")
+    sim.close()
 
 
 def test_march_live_calls_and_graphs():
@@ -93,5 +116,7 @@ def test_march_mode_argument_checked():
     with pytest.raises(lbm.LbmError):
         sim.set_stream_split(3)
     with pytest.raises(lbm.LbmError):
-        sim.set_march_segments(-1)
+        sim.set_march_shape(2, -1)
+    with pytest.raises(lbm.LbmError):
+        sim.set_march_shape(3, 0)
     sim.close()
