@@ -96,8 +96,10 @@ enum lbm_field {
 
 /* new Sim(nx, ny, nz) (src/sim.js:30-96): allocates the device state and sets
  * the reference's default parameters.  The state is zero until an init call.
- * At most 2^30 cells; lbm_step with float32 storage takes at most 2^29 (a
- * scene of 160 GB or more; the reference's typed arrays end near 2^26). */
+ * At most 2^30 cells; lbm_step with float32 storage addresses two planes of
+ * populations with 32-bit offsets, which takes grids whose planes (nx * ny)
+ * hold up to about 7.9 M cells (any nz; the reference's typed arrays end near
+ * 2^26 cells in all) and refuses larger planes with LBM_ERR_ARG. */
 int lbm_create(int nx, int ny, int nz, lbm_sim **out);
 void lbm_destroy(lbm_sim *sim);
 /* Message for the last failed call on this handle (or the last failed
