@@ -228,12 +228,18 @@ __global__ __launch_bounds__(RPW * kMarchCells) __attribute__((amdgpu_waves_per_
 inline bool march_fits(const Geom &g) { return g.nx >= 3 && g.nx <= kMarchCells && g.ny >= 3; }
 
 // Launch shape of the march with RPW rows per workgroup: row units per XCD
-// (cols8) and plane segments (nseg), about 4096 / RPW waves (sixteen per CU on
-// 256 CUs) unless nseg is given.
+// (cols8) and plane segments (nseg).  Unless nseg is given, segments of about
+// eight planes, and at least 4096 / RPW waves (sixteen per CU on 256 CUs):
+// short segments cost a little redundant phase work at their ends (two extra
+// planes of phi(t+1) per segment) but keep many independent workgroups in
+// flight (256^3 on MI355X, two rows per workgroup: 1.131 ms per step with 8
+// segments, 1.093-1.102 with 32 or 48, 1.101-1.110 with 64-128).
 inline void march_shape(const Geom &g, int rpw, int nseg_req, int &cols8, int &nseg) {
   cols8 = ((g.ny + rpw - 1) / rpw + 7) / 8;
   const int wgs = 1024 / rpw;
-  nseg = nseg_req > 0 ? nseg_req : (wgs + 8 * cols8 - 1) / (8 * cols8);
+  const int by_waves = (wgs + 8 * cols8 - 1) / (8 * cols8);
+  const int by_planes = (g.nz + 7) / 8;
+  nseg = nseg_req > 0 ? nseg_req : (by_planes > by_waves ? by_planes : by_waves);
   if (nseg < 1) nseg = 1;
   if (nseg > g.nz) nseg = g.nz;
 }

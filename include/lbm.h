@@ -276,7 +276,7 @@ int lbm_batch_set_stream_lanes(lbm_batch *batch, int cells_per_lane);
  * bulk/band-row split (lbm_set_stream_split, lbm_batch_set_stream_split),
  * for one float32 simulation whose rows fit it the row march in three
  * workgroup shapes (lbm_set_march_shape: two rows and the default segments,
- * one row, two rows and twice the segments), and steps per captured graph
+ * one row, two rows and half the segments), and steps per captured graph
  * (0, 2, 8, 32, those below trial_steps - 1) -- over trial_steps real
  * coupled steps each, after an untimed lead-in of 2 steps, keeps the fastest
  * and reports it.  Every candidate computes the same state, so the trials
@@ -460,8 +460,8 @@ int lbm_set_stream_lanes(lbm_sim *sim, int cells_per_lane);
 int lbm_set_stream_split(lbm_sim *sim, int mode);
 /* Workgroup shape of the row march (mode 2 above): each workgroup marches
  * `rows` rows (1 or 2; default 2) through nz / segments planes; segments 0
- * (default) chooses about 4096 / rows waves in all.  No reference
- * counterpart; results do not depend on it. */
+ * (default) chooses segments of about eight planes, and at least 4096 / rows
+ * waves in all.  No reference counterpart; results do not depend on it. */
 int lbm_set_march_shape(lbm_sim *sim, int rows, int segments);
 
 /* Measured device bandwidth of a plain copy: copies `bytes` (rounded down to 16)

@@ -20,6 +20,7 @@ MARCH_SCENES = [
     (70, 37, 45, 20.0, (35.3, 18.2, 22.6, 6.5, 2.5, -0.03)),  # rows of 70: two waves, one partial
     (128, 64, 33, 30.0, (64.0, 32.0, 16.0, 9.0, 1.5, 0.04)),  # the sphere's box meets the planes' segments
     (24, 24, 24, 12.0, None),                                 # no sphere: no band box
+    (64, 48, 40, 20.0, (6.2, 7.1, 5.8, 4.5, 2.0, -0.02)),     # box clipped at three grid faces
 ]
 CALLS = [5, 12, 1, 9]
 
