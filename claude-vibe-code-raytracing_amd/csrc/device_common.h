@@ -91,6 +91,7 @@ constexpr int kErrListOverflow = 1;
 constexpr int kErrBandEscape = 2;
 constexpr int kErrFreshCapacity = 4;
 constexpr int kErrBandTiles = 8;  // a band-tile launch did not cover every band-row tile
+constexpr int kErrLinksUnsorted = 16;  // link list too long for k_sort_apply_links
 
 // Tiles of 256 consecutive cells in the x-fastest order (four waves of 64),
 // launched on a 1-D grid in XCD-aware order.

@@ -218,6 +218,14 @@ __device__ __forceinline__ int lower_bound_i32(const int32_t *a, int n, int32_t 
   return lo;
 }
 
+// Neighbour n = 0..25 of a cell in the reference's visiting order (dz, dy, dx
+// each from -1 to 1, the cell itself skipped): its cell index offset.  The
+// first 13 are the neighbours with a smaller index.
+__device__ __forceinline__ int64_t nb_offset(const Geom &g, int n) {
+  const int m = n < 13 ? n : n + 1;
+  return (int64_t)(m / 9 - 1) * g.nxny + (int64_t)((m / 3) % 3 - 1) * g.nx + (m % 3 - 1);
+}
+
 // handleFreshCells (src/sim.js:257-310) over the sorted fresh list, in one
 // workgroup.  The reference updates cells in place in index order, so a fresh
 // cell averages the already-reseeded values of earlier fresh neighbours and the
@@ -236,34 +244,43 @@ __device__ void fresh_block(Geom g, double rhoG, double rhoL, const int32_t *__r
                             float *ux, float *uy, float *uz, typename Codec<KIND>::T *f,
                             typename Codec<KIND>::T *h) {
   using C = Codec<KIND>;
-  __shared__ int sh_level[4096];
+  constexpr int kLds = 4096;
+  __shared__ int sh_level[kLds];
+  __shared__ int32_t sh_list[kLds];
   __shared__ int sh_flag;
   if (count == 0) return;
   const int tid = threadIdx.x;
-  int *lv = count <= 4096 ? sh_level : level_scratch;
+  int *lv = count <= kLds ? sh_level : level_scratch;
+  const int32_t *sl = list;
+  if (count <= kLds) {
+    for (int c = tid; c < count; c += blockDim.x) sh_list[c] = list[c];
+    sl = sh_list;
+  }
   for (int c = tid; c < count; c += blockDim.x) lv[c] = 0;
   __threadfence_block();
   __syncthreads();
-  // Longest-path levels by repeated relaxation (terminates after depth+1 rounds).
+  // Longest-path levels by repeated relaxation (terminates after depth+1
+  // rounds).  The 13 earlier neighbours' tags are loaded together, so a round
+  // waits for memory once per cell.
   for (;;) {
     if (tid == 0) sh_flag = 0;
     __syncthreads();
     for (int c = tid; c < count; c += blockDim.x) {
-      const int32_t i = list[c];
-      const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / g.nxny);
+      const int32_t i = sl[c];
+      uint8_t tg[13];
+#pragma unroll
+      for (int n = 0; n < 13; n++) tg[n] = tag[i + nb_offset(g, n)];
       int L = 0;
-      for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-          for (int dx = -1; dx <= 1; dx++) {
-            if (!dx && !dy && !dz) continue;
-            const int64_t j = cell_index(g, x + dx, y + dy, z + dz);
-            if (j >= i || tag[j] != TAG_FLUID) continue;
-            const int q = lower_bound_i32(list, c, (int32_t)j);
-            if (q < c && list[q] == (int32_t)j) {
-              const int lq = lv[q] + 1;
-              if (lq > L) L = lq;
-            }
-          }
+#pragma unroll
+      for (int n = 0; n < 13; n++) {
+        if (tg[n] != TAG_FLUID) continue;
+        const int32_t j = (int32_t)(i + nb_offset(g, n));
+        const int q = lower_bound_i32(sl, c, j);
+        if (q < c && sl[q] == j) {
+          const int lq = lv[q] + 1;
+          if (lq > L) L = lq;
+        }
+      }
       if (L != lv[c]) {
         lv[c] = L;
         sh_flag = 1;
@@ -274,30 +291,49 @@ __device__ void fresh_block(Geom g, double rhoG, double rhoL, const int32_t *__r
     if (!sh_flag) break;
     __syncthreads();
   }
-  int maxL = 0;
-  for (int c = 0; c < count; c++) maxL = lv[c] > maxL ? lv[c] : maxL;
+  if (tid == 0) sh_flag = 0;
+  __syncthreads();
+  for (int c = tid; c < count; c += blockDim.x)
+    if (lv[c]) atomicMax(&sh_flag, lv[c]);
+  __syncthreads();
+  const int maxL = sh_flag;
   const double dRho = rhoL - rhoG;
   for (int L = 0; L <= maxL; L++) {
     for (int c = tid; c < count; c += blockDim.x) {
       if (lv[c] != L) continue;
-      const int32_t i = list[c];
-      const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / g.nxny);
+      const int32_t i = sl[c];
       double psum = 0, rsum = 0, vxs = 0, vys = 0, vzs = 0;
       int pcnt = 0;
-      for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-          for (int dx = -1; dx <= 1; dx++) {
-            if (!dx && !dy && !dz) continue;
-            const int64_t j = cell_index(g, x + dx, y + dy, z + dz);
-            if (tag[j] == TAG_FLUID) {
-              psum += (double)phi_w1[j];
-              rsum += (double)rho[j];
-              vxs += (double)ux[j];
-              vys += (double)uy[j];
-              vzs += (double)uz[j];
-              pcnt++;
-            }
-          }
+      // The neighbours one z plane at a time (9, 8 and 9 cells): their tags
+      // and fields are loaded together, then summed in the visiting order.
+#pragma unroll
+      for (int grp = 0; grp < 3; grp++) {
+        const int n0 = grp == 0 ? 0 : grp == 1 ? 9 : 17;
+        const int n1 = grp == 0 ? 9 : grp == 1 ? 17 : 26;
+        uint8_t tg[9];
+        float pv[9], rv[9], xv[9], yv[9], zv[9];
+#pragma unroll
+        for (int u = 0; u < 9; u++) {
+          if (n0 + u >= n1) continue;
+          const int64_t j = i + nb_offset(g, n0 + u);
+          tg[u] = tag[j];
+          pv[u] = phi_w1[j];
+          rv[u] = rho[j];
+          xv[u] = ux[j];
+          yv[u] = uy[j];
+          zv[u] = uz[j];
+        }
+#pragma unroll
+        for (int u = 0; u < 9; u++) {
+          if (n0 + u >= n1 || tg[u] != TAG_FLUID) continue;
+          psum += (double)pv[u];
+          rsum += (double)rv[u];
+          vxs += (double)xv[u];
+          vys += (double)yv[u];
+          vzs += (double)zv[u];
+          pcnt++;
+        }
+      }
       const double p = pcnt > 0 ? psum / pcnt : (double)phi_fb[i];
       const double r = pcnt > 0 ? rsum / pcnt : 1.0;
       const double vx = pcnt > 0 ? vxs / pcnt : 0;
@@ -325,7 +361,9 @@ __device__ void fresh_block(Geom g, double rhoG, double rhoL, const int32_t *__r
               w7(k) * pf);
       }
     }
-    __threadfence();
+    // The next level reads these cells from other threads of this workgroup
+    // only: the barrier's workgroup-scope fence makes the writes visible (an
+    // agent-scope fence would write back L2 at every level).
     __syncthreads();
   }
 }
@@ -462,8 +500,8 @@ __global__ __launch_bounds__(256) void k_collide_hydro(
 }
 
 // streamHydroAndMacro (src/sim.js:484-582).  With emit_links, each fluid->solid
-// link appends (key = i*27 + k, m = f_opp + f_k) so the host can replay the
-// applyImpulseAtCell calls in the reference's order.
+// link appends (key = i*27 + k, m = f_opp + f_k) in no particular order; sorted
+// by key, the list gives the applyImpulseAtCell calls in the reference's order.
 __global__ __launch_bounds__(256) void k_stream_hydro_macro(
     Geom g, Derived d, const float *__restrict__ f2, float *__restrict__ f,
     const float *__restrict__ h, float *__restrict__ rho, float *__restrict__ ux,
@@ -473,10 +511,34 @@ __global__ __launch_bounds__(256) void k_stream_hydro_macro(
     const float *__restrict__ gz, const float *__restrict__ lap, const float *__restrict__ bfx,
     const float *__restrict__ bfy, const float *__restrict__ bfz, int use_bf, int emit_links,
     int64_t *link_key, double *link_m, int link_cap, Ctrl *ctrl) {
-  LBM_CELL_XYZ
-  if (!interior(g, x, y, z)) return;
-  const int64_t i = cell_index(g, x, y, z);
-  if (tag[i] != TAG_FLUID) return;
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * 4 + threadIdx.y;
+  const int z = blockIdx.z;
+  bool active = x < g.nx && y < g.ny && interior(g, x, y, z);
+  const int64_t i = active ? cell_index(g, x, y, z) : 0;
+  active = active && tag[i] == TAG_FLUID;
+  int slot = 0;
+  if (emit_links) {
+    // One list reservation per workgroup: each lane counts its links (solid
+    // neighbours), takes its offset in LDS, and one thread reserves the
+    // total (one global atomic per workgroup instead of one per link).
+    __shared__ int blk_n, blk_base;
+    const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
+    if (lead) blk_n = 0;
+    __syncthreads();
+    int nl = 0;
+    if (active) {
+#pragma unroll
+      for (int k = 1; k < Q27; k++)
+        nl += tag[i - ((int64_t)EZ27[k] * g.nxny + (int64_t)EY27[k] * g.nx + EX27[k])] == TAG_SOLID;
+    }
+    if (nl) slot = atomicAdd(&blk_n, nl);
+    __syncthreads();
+    if (lead && blk_n) blk_base = atomicAdd(&ctrl->n_links, blk_n);
+    __syncthreads();
+    if (nl) slot += blk_base;
+  }
+  if (!active) return;
   const double rhoPhaseOld = d.rhoG + (double)phi[i] * d.dRho;
   const int64_t pe = pop_base(g, i);
   double pSumF = 0, mx = 0, my = 0, mz = 0;
@@ -498,13 +560,13 @@ __global__ __launch_bounds__(256) void k_stream_hydro_macro(
       fk = fopp + corr;
       if (st == TAG_SOLID && emit_links) {
         const double m = fopp + fk;
-        const int slot = atomicAdd(&ctrl->n_links, 1);
         if (slot < link_cap) {
           link_key[slot] = i * Q27 + k;
           link_m[slot] = m;
         } else {
           atomicOr(&ctrl->err, kErrListOverflow);
         }
+        slot++;
       }
     }
     f[pe + k * g.pb] = (float)fk;
@@ -560,23 +622,202 @@ __global__ __launch_bounds__(256) void k_count_solid(int64_t n, const uint8_t *_
   if (threadIdx.x == 0 && cnt) atomicAdd(&ctrl->n_solid, cnt);
 }
 
-// Dead-cell impulses rho_phase * u (src/solid.js:94-97) for the sorted dead list.
-__global__ __launch_bounds__(256) void k_dead_impulses(const int32_t *__restrict__ dead,
-                                                       const Ctrl *ctrl, int cap, double rhoG,
-                                                       double dRho, const float *__restrict__ phi,
-                                                       const float *__restrict__ ux,
-                                                       const float *__restrict__ uy,
-                                                       const float *__restrict__ uz,
-                                                       double *__restrict__ imp) {
-  int count = ctrl->n_dead;
-  if (count > cap) count = cap;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < count; c += gridDim.x * blockDim.x) {
-    const int32_t i = dead[c];
-    const double rhoPhase = rhoG + (double)phi[i] * dRho;
-    imp[3 * c + 0] = rhoPhase * (double)ux[i];
-    imp[3 * c + 1] = rhoPhase * (double)uy[i];
-    imp[3 * c + 2] = rhoPhase * (double)uz[i];
+// ---------------------------------------------------------------- sphere impulses on the device
+
+// The per-pass step's momentum exchange and dead-cell impulses applied to the
+// device sphere (ctrl->sphere) in the reference's order, so the host gets the
+// sphere back in one transfer instead of the link list.  Per chunk of 1024
+// records, every thread computes one record's six increments (impulse_terms,
+// the host's function, FP64) into LDS; then lanes 0..5 of the first wave add
+// them to Fx, Fy, Fz, Tx, Ty, Tz record after record.  Each sum sees the same
+// additions in the same order as applyImpulseAtCell called once per record
+// (src/solid.js:70-80), so the sphere ends bit for bit as the host's replay.
+
+constexpr int kApplyChunk = 1024;
+
+// rec(j, x, y, z, px, py, pz) yields record j; cnt records in order.
+template <class Rec>
+__device__ __forceinline__ void apply_impulses(Ctrl *ctrl, int cnt, const Rec &rec) {
+  __shared__ double terms[kApplyChunk][6];
+  if (cnt <= 0) return;
+  const int tid = threadIdx.x;
+  const lbm_sphere b = ctrl->sphere;
+  double *sums = &ctrl->sphere.Fx;  // Fx, Fy, Fz, Tx, Ty, Tz are consecutive
+  const int q = tid < 6 ? tid : 0;
+  double acc = sums[q];
+  for (int base = 0; base < cnt; base += kApplyChunk) {
+    const int j = base + tid;
+    if (j < cnt) {
+      int x, y, z;
+      double px, py, pz;
+      rec(j, x, y, z, px, py, pz);
+      impulse_terms(b, x, y, z, px, py, pz, terms[tid]);
+    }
+    __syncthreads();
+    if (tid < 64) {
+      // One dependent add per record; the loads of the next 16 records are
+      // issued before the adds of the current 16 so LDS latency stays hidden.
+      const int e = cnt - base < kApplyChunk ? cnt - base : kApplyChunk;
+      constexpr int U = 16;
+      int t = 0;
+      if (e >= U) {
+        double cur[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) cur[u] = terms[u][q];
+        for (; t + 2 * U <= e; t += U) {
+          double nxt[U];
+#pragma unroll
+          for (int u = 0; u < U; u++) nxt[u] = terms[t + U + u][q];
+#pragma unroll
+          for (int u = 0; u < U; u++) acc += cur[u];
+#pragma unroll
+          for (int u = 0; u < U; u++) cur[u] = nxt[u];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) acc += cur[u];
+        t += U;
+      }
+      for (; t < e; t++) acc += terms[t][q];
+    }
+    __syncthreads();
   }
+  if (tid < 6) sums[tid] = acc;
+}
+
+// Link (i, k) with momentum m (src/sim.js:530-535) is applyImpulseAtCell(
+// x - ex, y - ey, z - ez, -m ex, -m ey, -m ez) for cell i = (x, y, z).
+__device__ __forceinline__ void link_impulse(const Geom &g, int64_t key, double m, int &x, int &y,
+                                             int &z, double &px, double &py, double &pz) {
+  const int64_t i = key / Q27;
+  const int k = (int)(key - i * Q27);
+  const int ex = dir_comp(kEX27Bits, k), ey = dir_comp(kEY27Bits, k), ez = dir_comp(kEZ27Bits, k);
+  x = (int)(i % g.nx) - ex;
+  y = (int)((i / g.nx) % g.ny) - ey;
+  z = (int)(i / g.nxny) - ez;
+  px = -m * (double)ex;
+  py = -m * (double)ey;
+  pz = -m * (double)ez;
+}
+
+// Links sorted by key = cell * 27 + k: cells in index order, directions
+// k = 0..26 within a cell, the reference's loop order (src/sim.js:509-543).
+struct SortedLinkRecord {
+  Geom g;
+  const int64_t *keys;
+  const double *ms;
+  __device__ void operator()(int j, int &x, int &y, int &z, double &px, double &py,
+                             double &pz) const {
+    link_impulse(g, keys[j], ms[j], x, y, z, px, py, pz);
+  }
+};
+
+// Lists of sorted links (rocprim's radix sort ran first).
+__global__ __launch_bounds__(kApplyChunk) void k_apply_links(Geom g, const int64_t *__restrict__ keys,
+                                                             const double *__restrict__ ms,
+                                                             Ctrl *ctrl, int cnt) {
+  apply_impulses(ctrl, cnt, SortedLinkRecord{g, keys, ms});
+}
+
+// Lists up to kLinkSortLds links: sorted in LDS and applied by one workgroup.
+// The sort key is key << 13 | list position (keys stay below 2^51 for any
+// grid that fits in memory), so equal keys cannot occur and the list position
+// finds the momentum.  A longer list sets kErrLinksUnsorted and leaves the
+// sphere untouched; the host then sorts it with rocprim and runs k_apply_links.
+constexpr int kLinkSortLds = 8192;
+constexpr int kLinkPosBits = 13;
+
+// Sorts w[0..P) ascending (P a power of two) with the workgroup's threads, a
+// multiple of 64: a bitonic network whose stage (k, j) compares w[i] and
+// w[i + j] for pair p, i = (p / j) * 2j + p % j, ascending where i & k is 0.
+// Pairs p = 64 w' .. 64 w' + 63 of wave w' touch only entries 128 w' ..
+// 128 w' + 127 while j <= 64, so consecutive stages of that size need only
+// the wave's own ordering of its LDS accesses, not a workgroup barrier.
+__device__ __forceinline__ void bitonic_sort_lds(uint64_t *w, int P) {
+  const int tid = threadIdx.x;
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int p = tid; p < P / 2; p += blockDim.x) {
+        const int i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
+        const uint64_t a = w[i], b = w[i + j];
+        if ((a > b) == ((i & k) == 0)) {
+          w[i] = b;
+          w[i + j] = a;
+        }
+      }
+      const int j_next = j > 1 ? j >> 1 : k;  // after j = 1 comes (2k, k)
+      if (j > 64 || j_next > 64) {
+        __syncthreads();
+      } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
+  }
+  __syncthreads();
+}
+static_assert((1 << kLinkPosBits) == kLinkSortLds, "list position field");
+
+struct LdsLinkRecord {
+  Geom g;
+  const uint64_t *w;
+  const double *ms;
+  __device__ void operator()(int j, int &x, int &y, int &z, double &px, double &py,
+                             double &pz) const {
+    const uint64_t e = w[j];
+    link_impulse(g, (int64_t)(e >> kLinkPosBits), ms[e & (kLinkSortLds - 1)], x, y, z, px, py, pz);
+  }
+};
+
+__global__ __launch_bounds__(kApplyChunk) void k_sort_apply_links(Geom g,
+                                                                  const int64_t *__restrict__ keys,
+                                                                  const double *__restrict__ ms,
+                                                                  Ctrl *ctrl, int cap) {
+  __shared__ uint64_t w[kLinkSortLds];
+  const int tid = threadIdx.x;
+  const int cnt = ctrl->n_links < cap ? ctrl->n_links : cap;
+  if (cnt > kLinkSortLds) {
+    if (tid == 0) atomicOr(&ctrl->err, kErrLinksUnsorted);
+    return;
+  }
+  int P = 2;
+  while (P < cnt) P <<= 1;
+  for (int t = tid; t < P; t += blockDim.x)
+    w[t] = t < cnt ? ((uint64_t)keys[t] << kLinkPosBits) | (uint64_t)t : ~(uint64_t)0;
+  __syncthreads();
+  bitonic_sort_lds(w, P);
+  apply_impulses(ctrl, cnt, LdsLinkRecord{g, w, ms});
+}
+
+// absorbDeadCells (src/solid.js:87-101): the impulse rho_phase(phi) u of every
+// dead cell of the last retag (src/solid.js:94-97), in the list's index order.
+struct DeadRecord {
+  Geom g;
+  const int32_t *dead;
+  double rhoG, dRho;
+  const float *phi, *ux, *uy, *uz;
+  __device__ void operator()(int j, int &x, int &y, int &z, double &px, double &py,
+                             double &pz) const {
+    const int32_t i = dead[j];
+    const double rhoPhase = rhoG + (double)phi[i] * dRho;
+    x = (int)(i % g.nx);
+    y = (int)((i / g.nx) % g.ny);
+    z = (int)(i / g.nxny);
+    px = rhoPhase * (double)ux[i];
+    py = rhoPhase * (double)uy[i];
+    pz = rhoPhase * (double)uz[i];
+  }
+};
+
+__global__ __launch_bounds__(kApplyChunk) void k_absorb_dead(Geom g, const int32_t *__restrict__ dead,
+                                                             Ctrl *ctrl, int cap, double rhoG,
+                                                             double dRho,
+                                                             const float *__restrict__ phi,
+                                                             const float *__restrict__ ux,
+                                                             const float *__restrict__ uy,
+                                                             const float *__restrict__ uz) {
+  const int cnt = ctrl->n_dead < cap ? ctrl->n_dead : cap;
+  apply_impulses(ctrl, cnt, DeadRecord{g, dead, rhoG, dRho, phi, ux, uy, uz});
 }
 
 // packVolume (src/sim.js:586-600) for one cell: R = (clamp(phi, 0, 1) * 255) | 0,

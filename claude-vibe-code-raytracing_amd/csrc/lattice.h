@@ -32,6 +32,18 @@ constexpr int EY27[Q27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1
                            1, -1, 1, -1, -1, 1, 1, -1};
 constexpr int EZ27[Q27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1,
                            1, -1, -1, 1, 1, -1, 1, -1};
+// The components of e_k packed two bits each (component + 1), so that a
+// direction chosen at run time is decoded without a table in memory.
+constexpr uint64_t pack_dirs(const int (&e)[Q27]) {
+  uint64_t v = 0;
+  for (int k = 0; k < Q27; k++) v |= (uint64_t)(e[k] + 1) << (2 * k);
+  return v;
+}
+constexpr uint64_t kEX27Bits = pack_dirs(EX27), kEY27Bits = pack_dirs(EY27),
+                   kEZ27Bits = pack_dirs(EZ27);
+__host__ __device__ inline int dir_comp(uint64_t bits, int k) {
+  return (int)((bits >> (2 * k)) & 3u) - 1;
+}
 // src/lattices.js:66-69
 __host__ __device__ constexpr int opp27(int k) { return k == 0 ? 0 : ((k & 1) ? k + 1 : k - 1); }
 // src/lattices.js:52-57, stored as float32 (lattices.js:22).
@@ -73,18 +85,34 @@ __host__ __device__ inline void sphere_surface_velocity(const lbm_sphere &b, int
   out[2] = b.vz + (b.wx * ry - b.wy * rx);
 }
 
-// RigidSphere.applyImpulseAtCell (src/solid.js:70-80).
-__host__ __device__ inline void sphere_apply_impulse(lbm_sphere &b, int x, int y, int z, double px,
-                                                     double py, double pz) {
-  b.Fx += px;
-  b.Fy += py;
-  b.Fz += pz;
+// The six increments RigidSphere.applyImpulseAtCell (src/solid.js:70-80) adds
+// to Fx, Fy, Fz, Tx, Ty, Tz: the impulse p and the torque r x p about the centre.
+// They depend on the centre only, so a run of impulses may have its increments
+// computed in any order and summed one after the other.
+__host__ __device__ inline void impulse_terms(const lbm_sphere &b, int x, int y, int z, double px,
+                                              double py, double pz, double t[6]) {
   const double rx = x + 0.5 - b.cx;
   const double ry = y + 0.5 - b.cy;
   const double rz = z + 0.5 - b.cz;
-  b.Tx += ry * pz - rz * py;
-  b.Ty += rz * px - rx * pz;
-  b.Tz += rx * py - ry * px;
+  t[0] = px;
+  t[1] = py;
+  t[2] = pz;
+  t[3] = ry * pz - rz * py;
+  t[4] = rz * px - rx * pz;
+  t[5] = rx * py - ry * px;
+}
+
+// RigidSphere.applyImpulseAtCell (src/solid.js:70-80).
+__host__ __device__ inline void sphere_apply_impulse(lbm_sphere &b, int x, int y, int z, double px,
+                                                     double py, double pz) {
+  double t[6];
+  impulse_terms(b, x, y, z, px, py, pz, t);
+  b.Fx += t[0];
+  b.Fy += t[1];
+  b.Fz += t[2];
+  b.Tx += t[3];
+  b.Ty += t[4];
+  b.Tz += t[5];
 }
 
 // RigidSphere.recomputeMass (src/solid.js:39-44).

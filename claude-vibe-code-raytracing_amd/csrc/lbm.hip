@@ -15,6 +15,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprim/rocprim.hpp>
+
 #include "lbm.h"
 #include "device_common.h"
 #include "kernels_pass.h"
@@ -101,12 +103,21 @@ struct lbm_sim {
   int *level_scratch = nullptr;
   Ctrl *ctrl = nullptr;       // device
   Ctrl *ctrl_host = nullptr;  // pinned mirror
-  // Momentum-exchange link list (per-pass path).
+  // Momentum-exchange link list (per-pass path), its sorted copy and the
+  // sort's temporary storage.
   int64_t *link_key = nullptr;
   double *link_m = nullptr;
   int link_cap = 0;
-  double *imp = nullptr;  // dead-cell impulses
-  int imp_cap = 0;
+  int64_t *link_key2 = nullptr;
+  double *link_m2 = nullptr;
+  int link_cap2 = 0;
+  void *sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  // Pinned staging of sphere uploads that the per-pass step does not wait for,
+  // and an event after the last of them (the staging is reused once it fired).
+  lbm_sphere *sphere_pin = nullptr;
+  hipEvent_t sphere_ev = nullptr;
+  bool sphere_ev_armed = false;
   // Staging for host transfers and layout transposes.
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
@@ -319,18 +330,46 @@ static int ensure_links(lbm_sim *s, int64_t cap) {
   return LBM_OK;
 }
 
-static int ensure_imp(lbm_sim *s, int64_t cap) {
-  if (s->imp_cap >= cap) return LBM_OK;
-  dev_free(s->imp);
-  s->imp = nullptr;
-  s->imp_cap = 0;
-  HIP_TRY(s, dev_malloc(&s->imp, std::max<int64_t>(cap, 1) * 3 * sizeof(double)));
-  s->imp_cap = (int)cap;
+static int sync(lbm_sim *s) {
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
   return LBM_OK;
 }
 
-static int sync(lbm_sim *s) {
-  HIP_TRY(s, hipStreamSynchronize(s->stream));
+// Sorted-copy buffers and the radix sort's temporary storage for `cap` links.
+static int ensure_link_sort(lbm_sim *s, int64_t cap, int end_bit) {
+  if (s->link_cap2 < cap) {
+    dev_free(s->link_key2);
+    dev_free(s->link_m2);
+    s->link_key2 = nullptr;
+    s->link_m2 = nullptr;
+    s->link_cap2 = 0;
+    HIP_TRY(s, dev_malloc(&s->link_key2, cap * sizeof(int64_t)));
+    HIP_TRY(s, dev_malloc(&s->link_m2, cap * sizeof(double)));
+    s->link_cap2 = (int)cap;
+  }
+  size_t need = 0;
+  HIP_TRY(s, rocprim::radix_sort_pairs(nullptr, need, (uint64_t *)s->link_key,
+                                       (uint64_t *)s->link_key2, s->link_m, s->link_m2,
+                                       (unsigned)cap, 0, end_bit, s->stream));
+  if (need > s->sort_tmp_bytes) {
+    dev_free(s->sort_tmp);
+    s->sort_tmp = nullptr;
+    s->sort_tmp_bytes = 0;
+    HIP_TRY(s, dev_malloc(&s->sort_tmp, need));
+    s->sort_tmp_bytes = need;
+  }
+  return LBM_OK;
+}
+
+// Copies *b to the device sphere without waiting: through the pinned staging,
+// which the previous upload must have read first.
+static int upload_sphere_async(lbm_sim *s, const lbm_sphere *b) {
+  if (s->sphere_ev_armed) HIP_TRY(s, hipEventSynchronize(s->sphere_ev));
+  *s->sphere_pin = *b;
+  HIP_TRY(s, hipMemcpyAsync(&s->ctrl->sphere, s->sphere_pin, sizeof(lbm_sphere),
+                            hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(s, hipEventRecord(s->sphere_ev, s->stream));
+  s->sphere_ev_armed = true;
   return LBM_OK;
 }
 
@@ -455,7 +494,9 @@ extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
   if (e == hipSuccess)
     for (int b = 0; b < 2; b++) s->hbuf[b] = s->fbuf[b] + Q27 * s->g.pb;
   if (e == hipSuccess) e = host_malloc((void **)&s->ctrl_host, sizeof(Ctrl), hipHostMallocDefault);
+  if (e == hipSuccess) e = host_malloc((void **)&s->sphere_pin, sizeof(lbm_sphere), hipHostMallocDefault);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&s->sphere_ev, hipEventDisableTiming);
   if (e == hipSuccess) {
     // Zero the whole state so reads before an init are defined.
     const int64_t fb = S * sizeof(float);
@@ -488,12 +529,15 @@ extern "C" void lbm_destroy(lbm_sim *s) {
   void *ptrs[] = {s->fbuf[0], s->fbuf[1], s->phibuf[0], s->phibuf[1],
                   s->rho, s->ux, s->uy, s->uz, s->tag, s->tagPrev, s->usx, s->usy, s->usz, s->gx,
                   s->gy, s->gz, s->lap, s->bfx, s->bfy, s->bfz, s->fresh, s->dead,
-                  s->level_scratch, s->ctrl, s->link_key, s->link_m, s->imp, s->scratch,
+                  s->level_scratch, s->ctrl, s->link_key, s->link_m, s->link_key2, s->link_m2,
+                  s->sort_tmp, s->scratch,
                   s->mask_dev, s->mus_dev, s->partials, s->cf[0], s->cf[1], s->ch[0],
                   s->ch[1]};
   for (void *p : ptrs)
     if (p) dev_free(p);
   if (s->ctrl_host) host_free(s->ctrl_host);
+  if (s->sphere_pin) host_free(s->sphere_pin);
+  if (s->sphere_ev) hipEventDestroy(s->sphere_ev);
   s->graphs.clear();
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;
@@ -793,8 +837,96 @@ extern "C" int lbm_get_links(lbm_sim *s, lbm_link *out, int cap) {
   return LBM_OK;
 }
 
+// After a retag from sphere b, the solid cells are interior cells whose centre
+// lies within r of b's centre (sphere_contains): along each axis within
+// [c - r - 0.5, c + r - 0.5], and no more than the unit cubes around those
+// centres fit in a ball of radius r + sqrt(3)/2.  Both bounds get a margin
+// for rounding (one cell per side, radius r + 1).
+static int64_t sphere_cell_bound(const lbm_sim *s, const lbm_sphere &b) {
+  const double c[3] = {b.cx, b.cy, b.cz};
+  const int n[3] = {s->nx, s->ny, s->nz};
+  int64_t cells = 1;
+  for (int a = 0; a < 3; a++) {
+    double lo = 1, hi = n[a] - 2;  // interior cells
+    if (std::isfinite(c[a]) && std::isfinite(b.r)) {
+      lo = std::max(lo, std::floor(c[a] - b.r - 0.5) - 1);
+      hi = std::min(hi, std::ceil(c[a] + b.r - 0.5) + 1);
+    }
+    cells *= hi >= lo ? (int64_t)(hi - lo) + 1 : 0;
+  }
+  if (std::isfinite(b.r)) {
+    const double R = std::fabs(b.r) + 1;
+    const double ball = std::ceil(4.0 / 3.0 * 3.141592653589793 * R * R * R);
+    if (ball < (double)cells) cells = (int64_t)ball;
+  }
+  return cells;
+}
+
+// Sim.step with the library's sphere: the step's momentum exchange goes to the
+// device sphere in the reference's order (links sorted by cell and direction,
+// then applied one after the other), so the host waits once per step, for the
+// sphere coming back, instead of copying, sorting and replaying the link list.
+// Lists of up to kLinkSortLds links are sorted and applied by one kernel; a
+// longer one is sorted by rocprim's radix sort once its length is known (one
+// more wait).  The sphere ends as the host replay leaves it, bit for bit.
+static int sim_step_sphere(lbm_sim *s, lbm_sphere *solid) {
+  int rc;
+  if ((rc = upload_sphere_async(s, solid))) return rc;
+  if ((rc = retag_full(s, 1))) return rc;
+  s->need_full_retag = true;
+  // Every link has a solid source, and a cell sources at most 26 links; a list
+  // that still overflows sets kErrListOverflow and the step fails.
+  const int64_t bound = std::max<int64_t>(1, sphere_cell_bound(s, *solid) * 26);
+  if ((rc = ensure_links(s, bound))) return rc;
+  if ((rc = lbm_handle_fresh_cells(s))) return rc;
+  if ((rc = lbm_compute_phi_gradients(s))) return rc;
+  if ((rc = lbm_collide_phase(s))) return rc;
+  if ((rc = lbm_stream_phase(s))) return rc;
+  if ((rc = lbm_collide_hydro(s))) return rc;
+  const Derived d = derive(s->params);
+  HIP_TRY(s, hipMemsetAsync(&s->ctrl->n_links, 0, sizeof(int), s->stream));
+  k_stream_hydro_macro<<<cell_grid(s), kCellBlock, 0, s->stream>>>(
+      s->g, d, F2(s), F(s), H(s), s->rho, s->ux, s->uy, s->uz, PHI(s), s->tag, s->usx, s->usy,
+      s->usz, s->gx, s->gy, s->gz, s->lap, s->bfx, s->bfy, s->bfz, s->bodyF_armed ? 1 : 0, 1,
+      s->link_key, s->link_m, s->link_cap, s->ctrl);
+  if ((rc = check_launch(s, "k_stream_hydro_macro"))) return rc;
+  if (s->bodyF_armed) {
+    const int64_t fb = s->g.S * sizeof(float);
+    HIP_TRY(s, hipMemsetAsync(s->bfx, 0, fb, s->stream));
+    HIP_TRY(s, hipMemsetAsync(s->bfy, 0, fb, s->stream));
+    HIP_TRY(s, hipMemsetAsync(s->bfz, 0, fb, s->stream));
+    s->bodyF_armed = false;
+  }
+  k_sort_apply_links<<<1, kApplyChunk, 0, s->stream>>>(s->g, s->link_key, s->link_m, s->ctrl,
+                                                       s->link_cap);
+  if ((rc = check_launch(s, "k_sort_apply_links"))) return rc;
+  if ((rc = read_ctrl(s))) return rc;
+  if (s->ctrl_host->n_links > s->link_cap || (s->ctrl_host->err & kErrListOverflow))
+    return fail(s, LBM_ERR_CAPACITY, "momentum-exchange link list overflowed");
+  if (s->ctrl_host->err & kErrLinksUnsorted) {
+    const int cnt = s->ctrl_host->n_links;
+    // Keys (cell * 27 + k) are below 2^end_bit.
+    int end_bit = 1;
+    while (end_bit < 62 && ((int64_t)1 << end_bit) <= s->g.n * Q27) end_bit++;
+    if ((rc = ensure_link_sort(s, cnt, end_bit))) return rc;
+    size_t tmp = s->sort_tmp_bytes;
+    HIP_TRY(s, rocprim::radix_sort_pairs(s->sort_tmp, tmp, (uint64_t *)s->link_key,
+                                         (uint64_t *)s->link_key2, s->link_m, s->link_m2,
+                                         (unsigned)cnt, 0, end_bit, s->stream));
+    k_apply_links<<<1, kApplyChunk, 0, s->stream>>>(s->g, s->link_key2, s->link_m2, s->ctrl, cnt);
+    if ((rc = check_launch(s, "k_apply_links"))) return rc;
+    s->ctrl_host->err &= ~kErrLinksUnsorted;
+    HIP_TRY(s, hipMemcpyAsync(&s->ctrl->err, &s->ctrl_host->err, sizeof(int),
+                              hipMemcpyHostToDevice, s->stream));
+    if ((rc = read_ctrl(s))) return rc;
+  }
+  *solid = s->ctrl_host->sphere;
+  return LBM_OK;
+}
+
 extern "C" int lbm_sim_step(lbm_sim *s, lbm_sphere *solid) {
   CHECK_SIM_F32(s);
+  if (solid) return sim_step_sphere(s, solid);
   int rc;
   if ((rc = lbm_retag_cells(s, solid))) return rc;
   if ((rc = lbm_handle_fresh_cells(s))) return rc;
@@ -805,33 +937,21 @@ extern "C" int lbm_sim_step(lbm_sim *s, lbm_sphere *solid) {
   return lbm_stream_hydro_macro(s, solid);
 }
 
+// RigidSphere.absorbDeadCells (src/solid.js:87-101): the dead cells' impulses
+// applied to the sphere on the device in the list's index order (the host's
+// function, FP64), and the sphere read back in one transfer.
 extern "C" int lbm_absorb_dead_cells(lbm_sim *s, lbm_sphere *solid) {
   CHECK_SIM(s);
   CHECK_NOT_LIVE(s);
   if (!solid) return fail(s, LBM_ERR_ARG, "solid is NULL");
-  int rc = read_ctrl(s);
-  if (rc) return rc;
-  const int cnt = std::min(s->ctrl_host->n_dead, s->list_cap);
-  if (cnt == 0) return LBM_OK;
-  if ((rc = ensure_imp(s, cnt))) return rc;
+  int rc;
+  if ((rc = upload_sphere_async(s, solid))) return rc;
   const lbm_params &p = s->params;
-  k_dead_impulses<<<(cnt + 255) / 256, 256, 0, s->stream>>>(s->dead, s->ctrl, s->list_cap, p.rhoG,
-                                                            p.rhoL - p.rhoG, PHI(s), s->ux, s->uy,
-                                                            s->uz, s->imp);
-  if ((rc = check_launch(s, "k_dead_impulses"))) return rc;
-  std::vector<int32_t> idx(cnt);
-  std::vector<double> imp(3 * (size_t)cnt);
-  HIP_TRY(s, hipMemcpyAsync(idx.data(), s->dead, cnt * sizeof(int32_t), hipMemcpyDeviceToHost,
-                            s->stream));
-  HIP_TRY(s, hipMemcpyAsync(imp.data(), s->imp, cnt * 3 * sizeof(double), hipMemcpyDeviceToHost,
-                            s->stream));
-  if ((rc = sync(s))) return rc;
-  // src/solid.js:90-99: index order.
-  for (int c = 0; c < cnt; c++) {
-    const int64_t i = idx[c];
-    const int x = (int)(i % s->nx), y = (int)((i / s->nx) % s->ny), z = (int)(i / ((int64_t)s->nx * s->ny));
-    sphere_apply_impulse(*solid, x, y, z, imp[3 * c], imp[3 * c + 1], imp[3 * c + 2]);
-  }
+  k_absorb_dead<<<1, kApplyChunk, 0, s->stream>>>(s->g, s->dead, s->ctrl, s->list_cap, p.rhoG,
+                                                  p.rhoL - p.rhoG, PHI(s), s->ux, s->uy, s->uz);
+  if ((rc = check_launch(s, "k_absorb_dead"))) return rc;
+  if ((rc = read_ctrl(s))) return rc;
+  *solid = s->ctrl_host->sphere;
   return LBM_OK;
 }
 

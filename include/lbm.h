@@ -167,11 +167,15 @@ int lbm_stream_hydro_macro_links(lbm_sim *sim, int *count);
 /* Copies min(count, cap) links kept by the last lbm_stream_hydro_macro_links. */
 int lbm_get_links(lbm_sim *sim, lbm_link *out, int cap);
 
-/* Sim.step(solid) (src/sim.js:202-210): the seven passes above. */
+/* Sim.step(solid) (src/sim.js:202-210): the seven passes above.  With a
+ * sphere, the step's link impulses are applied on the device, sorted by
+ * (cell, direction) and added one after the other, and *solid is read back
+ * once: the same bits as the host replay of lbm_stream_hydro_macro. */
 int lbm_sim_step(lbm_sim *sim, lbm_sphere *solid);
 
 /* RigidSphere.absorbDeadCells(sim) (src/solid.js:87-101): adds each dead cell's
- * impulse (cells that became solid at the last retag), in index order. */
+ * impulse (cells that became solid at the last retag), in index order, on the
+ * device; *solid is uploaded and read back. */
 int lbm_absorb_dead_cells(lbm_sim *sim, lbm_sphere *solid);
 
 /* ------------------------------------------------------------------ coupled */

@@ -79,6 +79,75 @@ def test_per_pass_absorb_and_pack():
     assert np.array_equal(sim.pack(), A["packVolume"])
 
 
+def host_replay_step(sim, b):
+    """Sim.step + absorbDeadCells with the impulses replayed on the host, one
+    applyImpulseAtCell call per link in the reference's order
+    (src/sim.js:509-543, src/solid.js:87-101)."""
+    L = lbm.lib()
+    sim.retag(b)
+    sim.fresh()
+    sim.gradients()
+    sim.collide_phase()
+    sim.stream_phase()
+    sim.collide_hydro()
+    links = sim.stream_hydro_macro_links()
+    for (x, y, z, k, px, py, pz) in links:
+        L.lbm_sphere_apply_impulse_at_cell(b, x, y, z, px, py, pz)
+    p = sim.params()
+    phi, ux, uy, uz = (sim.get(k) for k in ("phi", "ux", "uy", "uz"))
+    dead = sim.dead_cells()
+    for i in dead:
+        i = int(i)
+        rho = p["rhoG"] + float(phi[i]) * (p["rhoL"] - p["rhoG"])
+        x, y, z = i % sim.nx, (i // sim.nx) % sim.ny, i // (sim.nx * sim.ny)
+        L.lbm_sphere_apply_impulse_at_cell(b, x, y, z, rho * float(ux[i]), rho * float(uy[i]),
+                                           rho * float(uz[i]))
+    return len(links), len(dead)
+
+
+@pytest.mark.parametrize("case", [
+    # ~7.4k links: sorted in LDS
+    ((48, 40, 36), (24.3, 18.2, 17.4, 8.0), (1, 8192)),
+    # ~25-30k links, past the LDS sort: rocprim's radix sort orders the list.
+    # The sphere starts clipped at the x = 0 face (the integrator then moves
+    # it inside the walls, src/solid.js:198-205).
+    ((64, 48, 56), (12.3, 22.7, 27.1, 16.0), (8193, 10 ** 9)),
+], ids=["lds-sort", "radix-sort"])
+def test_per_pass_device_impulses_equal_host_replay(case):
+    """lbm_sim_step and lbm_absorb_dead_cells apply the step's impulses to the
+    sphere on the device; the sphere must equal the host replay of the same
+    ordered impulses bit for bit, for a large, fast sphere (link lists of many
+    1024-record chunks)."""
+    (nx, ny, nz), (cx, cy, cz, r), (lo, hi) = case
+    announce("device impulses vs host replay", (nx, ny, nz), r)
+
+    def setup():
+        sim = lbm.Sim(nx, ny, nz)
+        sim.set_params(tau=0.6, tauPhi=0.6, sigma=0.01, gravity=1e-4, rhoL=1.0, rhoG=0.1, W=4.0)
+        sim.init_flat(ny // 2)
+        b = lbm.make_sphere(nx, ny, nz, cx, cy, cz, r, 1.8)
+        b.vx, b.vy, b.vz = 0.35, -0.2, 0.15
+        b.wx, b.wy, b.wz = 0.02, -0.03, 0.01
+        return sim, b
+
+    s1, b1 = setup()
+    s2, b2 = setup()
+    counts = []
+    for t in range(6):
+        s1.step(b1)
+        s1.absorb_dead(b1)
+        nl, nd = host_replay_step(s2, b2)
+        counts.append(nl)
+        assert b1.as_list() == b2.as_list(), f"sphere differs at step {t}"
+        for k in ("phi", "rho", "ux", "uy", "uz", "tag"):
+            assert same_values(s1.get(k), s2.get(k)), (t, k)
+        lbm.sphere_step(b1, 1.0, 1e-4)
+        lbm.sphere_step(b2, 1.0, 1e-4)
+    assert lo <= min(counts) and max(counts) <= hi, counts
+    s1.close()
+    s2.close()
+
+
 def run_golden(name, sim, b, nsteps, fields, fused, sphere_exact=True, field_tol=None):
     """Steps sim one call per step against the reference's recorded run: the
     sphere every step, the fresh and dead cell lists every step (when recorded),
