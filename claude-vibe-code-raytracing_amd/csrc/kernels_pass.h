@@ -718,54 +718,113 @@ __global__ __launch_bounds__(kApplyChunk) void k_apply_links(Geom g, const int64
   apply_impulses(ctrl, cnt, SortedLinkRecord{g, keys, ms});
 }
 
-// Lists up to kLinkSortLds links: sorted in LDS and applied by one workgroup.
-// The sort key is key << 13 | list position (keys stay below 2^51 for any
-// grid that fits in memory), so equal keys cannot occur and the list position
-// finds the momentum.  A longer list sets kErrLinksUnsorted and leaves the
-// sphere untouched; the host then sorts it with rocprim and runs k_apply_links.
+// Lists up to kLinkSortLds links whose keys fit 32 bits (grids below 2^32 / 27
+// cells) are sorted in LDS and applied by one workgroup.  A longer list, or a
+// larger grid, sets kErrLinksUnsorted and leaves the sphere untouched; the
+// host then sorts it with rocprim and runs k_apply_links.
 constexpr int kLinkSortLds = 8192;
-constexpr int kLinkPosBits = 13;
+constexpr int kRadixBits = 4, kRadixDigits = 1 << kRadixBits;
+constexpr int kSortWaves = kApplyChunk / 64;
+constexpr int kSortRounds = kLinkSortLds / kApplyChunk;  // list elements per thread
+constexpr int kSortSlots = kSortRounds * kSortWaves;     // (round, wave) pairs
+static_assert(kRadixDigits * kSortSlots == 2 * kApplyChunk, "two counts per thread in the scan");
 
-// Sorts w[0..P) ascending (P a power of two) with the workgroup's threads, a
-// multiple of 64: a bitonic network whose stage (k, j) compares w[i] and
-// w[i + j] for pair p, i = (p / j) * 2j + p % j, ascending where i & k is 0.
-// Pairs p = 64 w' .. 64 w' + 63 of wave w' touch only entries 128 w' ..
-// 128 w' + 127 while j <= 64, so consecutive stages of that size need only
-// the wave's own ordering of its LDS accesses, not a workgroup barrier.
-__device__ __forceinline__ void bitonic_sort_lds(uint64_t *w, int P) {
-  const int tid = threadIdx.x;
-  for (int k = 2; k <= P; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int p = tid; p < P / 2; p += blockDim.x) {
-        const int i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
-        const uint64_t a = w[i], b = w[i + j];
-        if ((a > b) == ((i & k) == 0)) {
-          w[i] = b;
-          w[i + j] = a;
-        }
-      }
-      const int j_next = j > 1 ? j >> 1 : k;  // after j = 1 comes (2k, k)
-      if (j > 64 || j_next > 64) {
-        __syncthreads();
-      } else {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      }
+struct LinkSortLds {
+  uint32_t key[2][kLinkSortLds];
+  uint16_t pos[2][kLinkSortLds];       // list position of the key (its momentum)
+  int cnt[kRadixDigits][kSortSlots];   // digit-major, then (round, wave)
+  int wsum[kSortWaves];
+};
+
+// Exclusive prefix sums, in place, of the 2 * kApplyChunk counts c (two
+// consecutive counts per thread).
+__device__ __forceinline__ void block_scan_pairs(int *c, int *wsum) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int a = c[2 * tid], b = c[2 * tid + 1];
+  int s = a + b;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(s, o, 64);
+    if (lane >= o) s += v;
+  }
+  if (lane == 63) wsum[wave] = s;
+  __syncthreads();
+  if (wave == 0) {
+    int t = lane < kSortWaves ? wsum[lane] : 0;
+#pragma unroll
+    for (int o = 1; o < kSortWaves; o <<= 1) {
+      const int v = __shfl_up(t, o, 64);
+      if (lane >= o) t += v;
     }
+    if (lane < kSortWaves) wsum[lane] = t;
   }
   __syncthreads();
+  const int before = (wave > 0 ? wsum[wave - 1] : 0) + s - (a + b);
+  c[2 * tid] = before;
+  c[2 * tid + 1] = before + a;
+  __syncthreads();
 }
-static_assert((1 << kLinkPosBits) == kLinkSortLds, "list position field");
+
+// Least-significant-digit radix sort of the first cnt keys of L.key[0] (and
+// their positions) over their low kb bits, four bits per pass.  A pass is
+// stable: element e = r * kApplyChunk + tid of round r goes after every
+// element of a smaller digit and after the elements of its digit in earlier
+// rounds, waves and lanes (ranks within a wave from ballots of the digit's
+// bits).  Returns the buffer holding the sorted list.
+__device__ __forceinline__ int radix_sort_lds(LinkSortLds &L, int cnt, int kb) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint64_t below = lane ? (~(uint64_t)0 >> (64 - lane)) : 0;  // lanes under this one
+  int src = 0;
+  for (int shift = 0; shift < kb; shift += kRadixBits) {
+    int *cnt_flat = &L.cnt[0][0];
+    for (int i = tid; i < kRadixDigits * kSortSlots; i += kApplyChunk) cnt_flat[i] = 0;
+    __syncthreads();
+    uint32_t dig[kSortRounds];
+    int rank[kSortRounds];
+#pragma unroll
+    for (int r = 0; r < kSortRounds; r++) {
+      dig[r] = 0;
+      rank[r] = 0;
+      if (r * kApplyChunk >= cnt) continue;  // the same for every thread
+      const int e = r * kApplyChunk + tid;
+      const bool v = e < cnt;
+      const uint32_t d = v ? (L.key[src][e] >> shift) & (kRadixDigits - 1) : 0u;
+      uint64_t m = __ballot(v);
+#pragma unroll
+      for (int bit = 0; bit < kRadixBits; bit++) {
+        const bool on = (d >> bit) & 1u;
+        const uint64_t bb = __ballot(v && on);
+        m &= on ? bb : ~bb;
+      }
+      const int rk = __popcll(m & below);
+      if (v && rk == 0) L.cnt[d][r * kSortWaves + wave] = __popcll(m);
+      dig[r] = d;
+      rank[r] = rk;
+    }
+    __syncthreads();
+    block_scan_pairs(cnt_flat, L.wsum);
+#pragma unroll
+    for (int r = 0; r < kSortRounds; r++) {
+      const int e = r * kApplyChunk + tid;
+      if (e >= cnt) continue;
+      const int dst = L.cnt[dig[r]][r * kSortWaves + wave] + rank[r];
+      L.key[src ^ 1][dst] = L.key[src][e];
+      L.pos[src ^ 1][dst] = L.pos[src][e];
+    }
+    __syncthreads();
+    src ^= 1;
+  }
+  return src;
+}
 
 struct LdsLinkRecord {
   Geom g;
-  const uint64_t *w;
+  const uint32_t *key;
+  const uint16_t *pos;
   const double *ms;
   __device__ void operator()(int j, int &x, int &y, int &z, double &px, double &py,
                              double &pz) const {
-    const uint64_t e = w[j];
-    link_impulse(g, (int64_t)(e >> kLinkPosBits), ms[e & (kLinkSortLds - 1)], x, y, z, px, py, pz);
+    link_impulse(g, (int64_t)key[j], ms[pos[j]], x, y, z, px, py, pz);
   }
 };
 
@@ -773,20 +832,22 @@ __global__ __launch_bounds__(kApplyChunk) void k_sort_apply_links(Geom g,
                                                                   const int64_t *__restrict__ keys,
                                                                   const double *__restrict__ ms,
                                                                   Ctrl *ctrl, int cap) {
-  __shared__ uint64_t w[kLinkSortLds];
+  __shared__ LinkSortLds L;
   const int tid = threadIdx.x;
   const int cnt = ctrl->n_links < cap ? ctrl->n_links : cap;
-  if (cnt > kLinkSortLds) {
+  const uint64_t max_key = (uint64_t)g.n * Q27 - 1;
+  const int kb = max_key ? 64 - __clzll((long long)max_key) : 1;  // bits of the largest key
+  if (cnt > kLinkSortLds || kb > 32) {
     if (tid == 0) atomicOr(&ctrl->err, kErrLinksUnsorted);
     return;
   }
-  int P = 2;
-  while (P < cnt) P <<= 1;
-  for (int t = tid; t < P; t += blockDim.x)
-    w[t] = t < cnt ? ((uint64_t)keys[t] << kLinkPosBits) | (uint64_t)t : ~(uint64_t)0;
+  for (int t = tid; t < cnt; t += kApplyChunk) {
+    L.key[0][t] = (uint32_t)keys[t];
+    L.pos[0][t] = (uint16_t)t;
+  }
   __syncthreads();
-  bitonic_sort_lds(w, P);
-  apply_impulses(ctrl, cnt, LdsLinkRecord{g, w, ms});
+  const int b = radix_sort_lds(L, cnt, kb);
+  apply_impulses(ctrl, cnt, LdsLinkRecord{g, L.key[b], L.pos[b], ms});
 }
 
 // absorbDeadCells (src/solid.js:87-101): the impulse rho_phase(phi) u of every
