@@ -112,7 +112,9 @@ def host_replay_step(sim, b):
     # The sphere starts clipped at the x = 0 face (the integrator then moves
     # it inside the walls, src/solid.js:198-205).
     ((64, 48, 56), (12.3, 22.7, 27.1, 16.0), (8193, 10 ** 9)),
-], ids=["lds-sort", "radix-sort"])
+    # a sphere of one or two cells: lists of a few links, or none
+    ((20, 18, 22), (9.5, 8.5, 10.5, 0.8), (0, 64)),
+], ids=["lds-sort", "rocprim-sort", "tiny"])
 def test_per_pass_device_impulses_equal_host_replay(case):
     """lbm_sim_step and lbm_absorb_dead_cells apply the step's impulses to the
     sphere on the device; the sphere must equal the host replay of the same
