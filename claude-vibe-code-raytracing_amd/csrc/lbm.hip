@@ -78,6 +78,8 @@ struct lbm_sim {
   Geom g;
   int device = 0;
   hipStream_t stream = nullptr;
+  // Memory pool of the server that made the handle (null: plain allocations).
+  hipMemPool_t pool = nullptr;
   lbm_params params{};
   // Population buffers (row blocks of the 27 f and 7 h directions, one per step
   // parity; hbuf[b] points into fbuf[b]): fbuf[fcur] is the reference's f,
@@ -181,6 +183,61 @@ static hipError_t host_free(void *p) {
   return hipHostFree(p);
 }
 
+// Device memory of a handle.  A handle that a server made takes its buffers
+// from the server's memory pool, ordered on the handle's stream, and gives them
+// back the same way: hipFree waits for the device (about 2.6 ms a buffer while
+// a server steps other scenes, 80 ms for a scene's thirty buffers), and memory
+// freed into the pool is taken by the next scene without going back to the
+// driver.  An allocation waits for the handle's stream, so the buffer is ready
+// for any stream once it returns; a buffer is given back only when no queued
+// work uses it.
+template <typename T>
+static hipError_t sim_malloc(lbm_sim *s, T **p, size_t bytes) {
+  if (!s->pool) return dev_malloc(p, bytes);
+  std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
+  hipError_t e = hipMallocFromPoolAsync((void **)p, bytes, s->pool, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  return e;
+}
+static hipError_t sim_free(lbm_sim *s, void *p) {
+  if (!p) return hipSuccess;
+  if (!s->pool) return dev_free(p);
+  std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
+  return hipFreeAsync(p, s->stream);
+}
+
+// Small pinned host blocks (a handle's control-block mirror and sphere
+// staging) are kept for the next handle instead of released: hipHostFree waits
+// for the device as hipFree does.  At most kPinCacheMax blocks are kept.
+static std::mutex g_pin_mu;
+static std::multimap<size_t, void *> g_pin_cache;
+constexpr size_t kPinCacheMax = 4096;
+template <typename T>
+static hipError_t pin_small_take(T **p, size_t bytes) {
+  {
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    auto it = g_pin_cache.find(bytes);
+    if (it != g_pin_cache.end()) {
+      *p = (T *)it->second;
+      g_pin_cache.erase(it);
+      std::memset((void *)*p, 0, bytes);
+      return hipSuccess;
+    }
+  }
+  return host_malloc(p, bytes, hipHostMallocDefault);
+}
+static void pin_small_give(void *p, size_t bytes) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    if (g_pin_cache.size() < kPinCacheMax) {
+      g_pin_cache.emplace(bytes, p);
+      return;
+    }
+  }
+  host_free(p);
+}
+
 // ------------------------------------------------------------------ no-alloc scratch
 // Device memory of lbm_step_noalloc: the fused step's link-momentum partials
 // (6 doubles per 256-cell tile) in a file-scope device array instead of the
@@ -268,10 +325,10 @@ static float *PHIPREV(lbm_sim *s) { return s->phibuf[1 - s->phicur]; }
 
 static int ensure_scratch(lbm_sim *s, size_t bytes) {
   if (s->scratch_bytes >= bytes) return LBM_OK;
-  if (s->scratch) dev_free(s->scratch);
+  sim_free(s, s->scratch);
   s->scratch = nullptr;
   s->scratch_bytes = 0;
-  HIP_TRY(s, dev_malloc(&s->scratch, bytes));
+  HIP_TRY(s, sim_malloc(s, &s->scratch, bytes));
   s->scratch_bytes = bytes;
   return LBM_OK;
 }
@@ -319,13 +376,13 @@ static int ensure_f32(lbm_sim *s, hipStream_t st) {
 static int ensure_links(lbm_sim *s, int64_t cap) {
   if (cap > INT32_MAX) return fail(s, LBM_ERR_CAPACITY, "link list capacity exceeds 2^31");
   if (s->link_cap >= cap) return LBM_OK;
-  dev_free(s->link_key);
-  dev_free(s->link_m);
+  sim_free(s, s->link_key);
+  sim_free(s, s->link_m);
   s->link_key = nullptr;
   s->link_m = nullptr;
   s->link_cap = 0;
-  HIP_TRY(s, dev_malloc(&s->link_key, cap * sizeof(int64_t)));
-  HIP_TRY(s, dev_malloc(&s->link_m, cap * sizeof(double)));
+  HIP_TRY(s, sim_malloc(s, &s->link_key, cap * sizeof(int64_t)));
+  HIP_TRY(s, sim_malloc(s, &s->link_m, cap * sizeof(double)));
   s->link_cap = (int)cap;
   return LBM_OK;
 }
@@ -338,13 +395,13 @@ static int sync(lbm_sim *s) {
 // Sorted-copy buffers and the radix sort's temporary storage for `cap` links.
 static int ensure_link_sort(lbm_sim *s, int64_t cap, int end_bit) {
   if (s->link_cap2 < cap) {
-    dev_free(s->link_key2);
-    dev_free(s->link_m2);
+    sim_free(s, s->link_key2);
+    sim_free(s, s->link_m2);
     s->link_key2 = nullptr;
     s->link_m2 = nullptr;
     s->link_cap2 = 0;
-    HIP_TRY(s, dev_malloc(&s->link_key2, cap * sizeof(int64_t)));
-    HIP_TRY(s, dev_malloc(&s->link_m2, cap * sizeof(double)));
+    HIP_TRY(s, sim_malloc(s, &s->link_key2, cap * sizeof(int64_t)));
+    HIP_TRY(s, sim_malloc(s, &s->link_m2, cap * sizeof(double)));
     s->link_cap2 = (int)cap;
   }
   size_t need = 0;
@@ -352,10 +409,10 @@ static int ensure_link_sort(lbm_sim *s, int64_t cap, int end_bit) {
                                        (uint64_t *)s->link_key2, s->link_m, s->link_m2,
                                        (unsigned)cap, 0, end_bit, s->stream));
   if (need > s->sort_tmp_bytes) {
-    dev_free(s->sort_tmp);
+    sim_free(s, s->sort_tmp);
     s->sort_tmp = nullptr;
     s->sort_tmp_bytes = 0;
-    HIP_TRY(s, dev_malloc(&s->sort_tmp, need));
+    HIP_TRY(s, sim_malloc(s, &s->sort_tmp, need));
     s->sort_tmp_bytes = need;
   }
   return LBM_OK;
@@ -424,7 +481,9 @@ static void collect_timing(Timer &t) {
 
 // ------------------------------------------------------------------ lifetime
 
-extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
+// lbm_create, with the handle's device memory from `pool` when it is not null
+// (the serving loop's scenes).
+static int create_sim(int nx, int ny, int nz, hipMemPool_t pool, lbm_sim **out) {
   if (!out) return LBM_ERR_ARG;
   *out = nullptr;
   if (nx < 3 || ny < 3 || nz < 3) {
@@ -461,10 +520,11 @@ extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
   while (cap < n) cap <<= 1;
   s->list_cap = cap;
   const int64_t S = s->g.S;
-  hipError_t e = hipSuccess;
+  s->pool = pool;
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
 #define ALLOC(ptr, bytes)                                        \
   if (e == hipSuccess) {                                         \
-    e = dev_malloc((void **)&(ptr), (bytes));                     \
+    e = sim_malloc(s, (void **)&(ptr), (bytes));                  \
   }
   ALLOC(s->fbuf[0], S * kRowQ * sizeof(float));
   ALLOC(s->fbuf[1], S * kRowQ * sizeof(float));
@@ -493,9 +553,8 @@ extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
 #undef ALLOC
   if (e == hipSuccess)
     for (int b = 0; b < 2; b++) s->hbuf[b] = s->fbuf[b] + Q27 * s->g.pb;
-  if (e == hipSuccess) e = host_malloc((void **)&s->ctrl_host, sizeof(Ctrl), hipHostMallocDefault);
-  if (e == hipSuccess) e = host_malloc((void **)&s->sphere_pin, sizeof(lbm_sphere), hipHostMallocDefault);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = pin_small_take(&s->ctrl_host, sizeof(Ctrl));
+  if (e == hipSuccess) e = pin_small_take(&s->sphere_pin, sizeof(lbm_sphere));
   if (e == hipSuccess) e = hipEventCreateWithFlags(&s->sphere_ev, hipEventDisableTiming);
   if (e == hipSuccess) {
     // Zero the whole state so reads before an init are defined.
@@ -521,6 +580,10 @@ extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
   return LBM_OK;
 }
 
+extern "C" int lbm_create(int nx, int ny, int nz, lbm_sim **out) {
+  return create_sim(nx, ny, nz, nullptr, out);
+}
+
 extern "C" void lbm_destroy(lbm_sim *s) {
   if (!s) return;
   std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
@@ -533,10 +596,11 @@ extern "C" void lbm_destroy(lbm_sim *s) {
                   s->sort_tmp, s->scratch,
                   s->mask_dev, s->mus_dev, s->partials, s->cf[0], s->cf[1], s->ch[0],
                   s->ch[1]};
-  for (void *p : ptrs)
-    if (p) dev_free(p);
-  if (s->ctrl_host) host_free(s->ctrl_host);
-  if (s->sphere_pin) host_free(s->sphere_pin);
+  for (void *p : ptrs) sim_free(s, p);
+  // Pooled buffers go back to the pool once the stream reaches their frees.
+  if (s->pool && s->stream) hipStreamSynchronize(s->stream);
+  pin_small_give(s->ctrl_host, sizeof(Ctrl));
+  pin_small_give(s->sphere_pin, sizeof(lbm_sphere));
   if (s->sphere_ev) hipEventDestroy(s->sphere_ev);
   s->graphs.clear();
   if (s->stream) hipStreamDestroy(s->stream);
@@ -670,8 +734,8 @@ extern "C" int lbm_retag_cells_mask(lbm_sim *s, const uint8_t *mask, const float
   if (!mask || !us) return fail(s, LBM_ERR_ARG, "mask/us is NULL");
   const int64_t n = s->g.n;
   if (!s->mask_dev) {
-    HIP_TRY(s, dev_malloc(&s->mask_dev, n));
-    HIP_TRY(s, dev_malloc(&s->mus_dev, n * 3 * sizeof(float)));
+    HIP_TRY(s, sim_malloc(s, &s->mask_dev, n));
+    HIP_TRY(s, sim_malloc(s, &s->mus_dev, n * 3 * sizeof(float)));
   }
   HIP_TRY(s, hipMemcpyAsync(s->mask_dev, mask, n, hipMemcpyHostToDevice, s->stream));
   HIP_TRY(s, hipMemcpyAsync(s->mus_dev, us, n * 3 * sizeof(float), hipMemcpyHostToDevice, s->stream));
@@ -1077,8 +1141,8 @@ extern "C" int lbm_set_storage(lbm_sim *s, int kind) {
   if (kind != LBM_STORAGE_F32 && !s->cf[0]) {
     const int64_t S = s->g.S;
     for (int b = 0; b < 2; b++) {
-      HIP_TRY(s, dev_malloc(&s->cf[b], S * 27 * 2));
-      HIP_TRY(s, dev_malloc(&s->ch[b], S * 7 * 2));
+      HIP_TRY(s, sim_malloc(s, &s->cf[b], S * 27 * 2));
+      HIP_TRY(s, sim_malloc(s, &s->ch[b], S * 7 * 2));
       // The fused step never writes the populations of cells that are not
       // fluid: clear them so that what reads back there is defined (a zero
       // deviation) rather than whatever the allocation held.
