@@ -350,7 +350,9 @@ typedef struct lbm_server lbm_server;
 int64_t lbm_state_bytes(int nx, int ny, int nz, int storage);
 /* hbm_budget: device bytes the resident scenes may take; max_active: active
  * scene cap; frame_queue_bytes: unread frame bytes before the loop waits for
- * the reader (0: 1 GiB).  Starts the loop thread. */
+ * the reader (0: 1 GiB).  Starts the loop thread.  The scenes' device memory
+ * comes from a memory pool of the server's own, which keeps what finished
+ * scenes free for the next ones until lbm_server_destroy. */
 int lbm_server_create(int64_t hbm_budget, int max_active, int64_t frame_queue_bytes,
                       lbm_server **out);
 /* Stops the loop and frees every resident scene. */
