@@ -196,6 +196,12 @@ static hipError_t sim_malloc(lbm_sim *s, T **p, size_t bytes) {
   if (!s->pool) return dev_malloc(p, bytes);
   std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
   hipError_t e = hipMallocFromPoolAsync((void **)p, bytes, s->pool, s->stream);
+  if (e == hipSuccess && ((uintptr_t)*p & 255u) != 0) {
+    // The kernels assume 256-byte aligned buffers, as hipMalloc gives them.
+    (void)hipFreeAsync(*p, s->stream);
+    *p = nullptr;
+    e = hipErrorInvalidValue;
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   return e;
 }
