@@ -57,7 +57,7 @@ def drain(srv, ids, timeout_s=120):
     return got
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16"])
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
 def test_serve_same_alone_and_in_mix(storage):
     announce("serve alone vs mix", storage)
     budget = 8 << 30
@@ -108,6 +108,33 @@ def test_serve_admission_by_budget():
     for i in ids:
         assert [f[1] for f in got[i]] == [10, 20]
     assert st["lattice_updates"] == 20 * sum(sizes)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_serve_scene_in_reused_memory(storage):
+    """A server's scenes take device memory that finished scenes gave back
+    (its memory pool): a request served after others have come and gone, one
+    at a time, sends the frames it sends on a fresh server."""
+    announce("serve scene in reused memory", storage)
+    dims = [scene(i)[0] for i in (3, 4, 5)]
+    need = max(lbm.state_bytes(*d, storage) for d in dims)
+    srv = lbm.Server(4 << 30)
+    a = submit(srv, 3, 40, storage)
+    fresh = drain(srv, [a])[a]
+    srv.close()
+    # One scene at a time, the larger ones first, so the last one's buffers
+    # come from memory they wrote.
+    srv = lbm.Server(need + need // 4)
+    ids = [submit(srv, i, 20, storage) for i in (5, 4)]
+    a2 = submit(srv, 3, 40, storage)
+    got = drain(srv, ids + [a2])
+    st = srv.stats()
+    srv.close()
+    assert st["max_active_seen"] == 1 and st["completed"] == 3, st
+    assert [f[:3] for f in got[a2]] == [f[:3] for f in fresh]
+    for x, y in zip(got[a2], fresh):
+        assert np.array_equal(x[3], y[3]), x[1]
+        assert x[4] == y[4], x[1]
 
 
 def test_serve_rejects_oversized_request():
