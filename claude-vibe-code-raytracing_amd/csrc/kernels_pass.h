@@ -236,9 +236,12 @@ __device__ __forceinline__ int64_t nb_offset(const Geom &g, int n) {
 //
 // phi_fb: phiPrev read for the no-neighbour fallback; phi_w1/phi_w2: the two
 // writes of phi[i] = phiPrev[i] = p (they may alias); neighbours read phi_w1.
-// f and h receive the reseeded populations.
+// f and h receive the reseeded populations.  Inlined into its kernels: as a
+// called function its arguments and the caller's live registers went through
+// scratch memory.
 template <int KIND>
-__device__ void fresh_block(Geom g, double rhoG, double rhoL, const int32_t *__restrict__ list,
+__device__ __forceinline__ void fresh_block(Geom g, double rhoG, double rhoL,
+                                            const int32_t *__restrict__ list,
                             int count, int *level_scratch, const uint8_t *__restrict__ tag,
                             const float *phi_fb, float *phi_w1, float *phi_w2, float *rho,
                             float *ux, float *uy, float *uz, typename Codec<KIND>::T *f,
